@@ -1,0 +1,192 @@
+// Internal declarations of the HIP inverse-DFT library (libidft.so).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/idft.h"
+#include "../../include/idft_hip.h"
+
+namespace idft {
+
+// Error carried from a failing HIP call or an invalid state up to the C ABI.
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define IDFT_HIP(call)                                                              \
+  do {                                                                              \
+    hipError_t e_ = (call);                                                         \
+    if (e_ != hipSuccess)                                                           \
+      throw ::idft::Error(IDFT_EHIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// Owning device buffer (hipMalloc'd, freed on destruction).
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  explicit DevBuf(size_t count) { resize(count); }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr; n = 0;
+  }
+  // Grow-only reallocation; contents are not preserved.
+  void resize(size_t count) {
+    if (count <= n && p) return;
+    release();
+    if (count == 0) return;
+    hipError_t e = hipMalloc(&p, count * sizeof(T));
+    if (e != hipSuccess) { p = nullptr; throw Error(IDFT_ENOMEM, "hipMalloc failed"); }
+    n = count;
+  }
+  void upload(const T* h, size_t count, hipStream_t s) {
+    resize(count);
+    IDFT_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+  }
+  void download(T* h, size_t count, hipStream_t s) const {
+    IDFT_HIP(hipMemcpyAsync(h, p, count * sizeof(T), hipMemcpyDeviceToHost, s));
+  }
+};
+
+// Host-side finite-element tables of one (n_cells, p) discretisation.
+struct Tables {
+  int nc = 0, p = 0, n1 = 0, q1 = 0, ne = 0, nq = 0, N1 = 0, NV1 = 0;
+  int64_t N = 0, ncell = 0, NV = 0;
+  double L = 0, h = 0;
+  std::vector<double> gll_x, gll_w, gq_x, gq_w;
+  std::vector<double> Lq;      // [q1][n1] basis at parent Gauss points
+  std::vector<double> Lc;      // [4][n1]  basis at the child Gauss points
+  std::vector<double> M1, D1;  // [n1][n1] 1D mass and stiffness on [-1,1]
+  std::vector<double> Wq;      // [q1][3] child Q1 basis (3 nodes / parent axis) at parent points
+  std::vector<double> Wc;      // [4][3]  child Q1 basis at the child Gauss points
+  void init(const idft_desc& d);
+};
+
+bool desc_valid(const idft_desc* d);
+
+// ---- small dense linear algebra on the host (B x B, FP64)
+void sym_eig(int n, std::vector<double> A, std::vector<double>& evals, std::vector<double>& V);
+bool cholesky_lower(int n, std::vector<double>& S);
+uint64_t splitmix64(uint64_t& s);
+double rand_pm1(uint64_t& s);
+double fd_occ(double e, double mu, double T);
+double fd_occ_der(double e, double mu, double T);
+double fermi_level(const std::vector<double>& e, double ne, double T);
+
+// ---- device kernels (kernels.hip) -------------------------------------------
+// Geometry passed to kernels by value.
+struct Geo {
+  int nc, p, n1, N1, q1, NV1;
+  int64_t N, ncell;
+};
+
+// Constant tables (uploaded once per context).
+struct DevTables {
+  DevBuf<double> Lq;  // [p+3][p+1] parent basis at parent Gauss points
+  DevBuf<double> Lc;  // [4][p+1]   parent basis at child Gauss points
+  DevBuf<double> M1, D1;  // [p+1][p+1] 1D mass / stiffness on [-1,1]
+  DevBuf<double> Wq;  // [p+3][3] child Q1 basis (3 child nodes per parent axis) at parent points
+  DevBuf<double> Wc;  // [4][3]   child Q1 basis at child Gauss points
+};
+
+constexpr int kMaxCols = 80;  // widest block the cell apply handles in one pass
+
+// Per-column MINRES state slots: st[slot * B + col].
+enum MinresSlot {
+  MS_BETA, MS_OLDB, MS_DBAR, MS_EPSLN, MS_PHIBAR, MS_CS, MS_SN, MS_ALFA, MS_DONE, MS_IT, MS_TOL,
+  MS_BETA1, MS_CV, MS_CR1, MS_CA2, MS_CWV, MS_CWW1, MS_CWW, MS_CX, MS_COUNT
+};
+
+int64_t hamiltonian_tile_elems(int p);
+// H_e = K_e/2 + V_e(veff) for all cells in the MFMA tile layout (double and,
+// when h32 != nullptr, float).  jxw[q] is the (uniform) parent JxW.
+void launch_build_hamiltonian(const Geo& g, const DevTables& t, double hh, const double* veff,
+                              const double* jxw, double* h64, float* h32, hipStream_t s);
+// Y[node] += alpha * S[node] (H_e (S X)_e)[node], all 8 colours, B <= kMaxCols.
+template <typename T>
+void launch_cell_apply(const Geo& g, const T* H, const T* S, const T* X, T* Y, int B,
+                       double alpha, hipStream_t s);
+// Y = beta Y + ns[n] (a ca[j] X + b cb[j] Z); null pointers read as 1 (X/Z as 0).
+template <typename T>
+void launch_lincomb(int64_t N, int B, T* Y, double beta, const T* X, double a, const double* ca,
+                    const T* Z, double b, const double* cb, const double* ns, hipStream_t s);
+template <typename T>
+void launch_lincomb3(int64_t N, int B, T* Y, double beta, const T* X, double a, const double* ca,
+                     const T* Z, double b, const double* cb, const T* W, double c,
+                     const double* cc, const double* ns, hipStream_t s);
+void launch_convert_d2f(int64_t n, float* y, const double* x, hipStream_t s);
+void launch_convert_f2d(int64_t n, double* y, const float* x, hipStream_t s);
+int coldot_blocks(int64_t N, int B);
+// partial[blk][j] = sum_n w[n] X[n][j] Y[n][j] (w may be null); returns #blocks
+template <typename T>
+int launch_coldot_partial(int64_t N, int B, const T* X, const T* Y, const double* w,
+                          double* partial, hipStream_t s);
+void launch_sum_partials(int nblk, int B, const double* partial, double* out, hipStream_t s);
+int gram_partial_blocks(int64_t N, int B);
+// G = X^T Y (B x B)
+void launch_gram(int64_t N, int B, const double* X, const double* Y, double* partial, double* G,
+                 hipStream_t s);
+// Y = beta Y + ns[n] (X Q)[n]  (Q row-major B x B on device, X != Y)
+void launch_right_mult(int64_t N, int B, const double* X, const double* Q, double* Y,
+                       const double* ns, double beta, hipStream_t s);
+int launch_residual_partial(int64_t N, int B, const double* HX, const double* X,
+                            const double* ev, double* partial, hipStream_t s);
+void launch_vxc_to_quad(const Geo& g, const DevTables& t, const double* pot,
+                        const double* potBase, double* veff, hipStream_t s);
+// rho(q) = sum_j 2 occ[j] (sum_a N_a(q) S[n_a] X[n_a][j])^2
+void launch_density(const Geo& g, const DevTables& t, const double* X, const double* S, int B,
+                    const double* occ, double* rho, hipStream_t s);
+// u = w (rho_t - rho/2); partial[blk][4] = (loss, loss_unweighted, l1, int rho v)
+void launch_loss(int64_t nQ, int nq, const double* rho, const double* rhoT, const double* w,
+                 const double* veff, const double* jxw, double* u, double* partial,
+                 hipStream_t s);
+constexpr int kLossBlocks = 256;
+// out[cell][k] = sum_q wq[cell][q] A_{i_k}(q) B_{j_k}(q), pairs (i_k, j_k)
+void launch_pair_quad(const Geo& g, const DevTables& t, const double* A, const double* Bv, int B,
+                      const int* pairs, int np, const double* wq, int64_t wstride, double* out,
+                      hipStream_t s);
+void launch_copy_cols(int64_t N, int Bs, int off, int Bd, const double* src, double* dst,
+                      hipStream_t s);
+// out[q] = u[q] * jxw[q % nq]
+void launch_mul_q(int64_t nQ, int nq, const double* u, const double* jxw, double* out,
+                  hipStream_t s);
+// partial[blk][k] = sum_n w[n] X[n][i_k] Y[n][j_k] (w may be null)
+int launch_pairdot_partial(int64_t N, int B, const double* X, const double* Y, const double* w,
+                           const int* pairs, int np, double* partial, hipStream_t s);
+// Y += N^T diag(wq) N X over the cells of one colour
+void launch_quad_operator(const Geo& g, const DevTables& t, const double* X, int B,
+                          const double* wq, double* Y, int colour, hipStream_t s);
+// force[child node] += int N^child sum_j Psi_j P_j over one colour; mode 0 = child
+// Gauss points (weight jxwChild), mode 1 = parent Gauss points (weights jxwParent[q])
+void launch_force(const Geo& g, const DevTables& t, const double* Psi, const double* P, int B,
+                  int mode, double jxwChild, const double* jxwParent, double* force, int colour,
+                  hipStream_t s);
+void launch_zero_rows(int64_t N, int B, double* X, const unsigned char* mask, hipStream_t s);
+// MINRES
+void launch_minres_start(int B, int nblk, const double* partial, double* st, hipStream_t s);
+void launch_minres_alpha(int B, int nblk, const double* partial, double* st, hipStream_t s);
+template <typename T>
+int launch_minres_lanczos(int64_t N, int B, T* y, const T* r2, T* rnew, const double* D,
+                          const double* st, double* partial, hipStream_t s);
+void launch_minres_givens(int B, int nblk, const double* partial, double* st, hipStream_t s);
+template <typename T>
+void launch_minres_update(int64_t N, int B, const T* v, T* w1, const T* w, double* x,
+                          const double* st, hipStream_t s);
+void launch_copy_done(int B, const double* st, int* flags, hipStream_t s);
+
+}  // namespace idft

@@ -1,0 +1,1222 @@
+// Host orchestration of the MI355X inverse-DFT hot path: context set-up, the
+// ChFSI eigensolve, density/loss, the adjoint problem contract, MINRES and
+// the force.  All N-sized work runs in HIP kernels (kernels.hip) on the
+// context's stream; the host handles only B x B dense algebra and scalars.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <set>
+#include <type_traits>
+
+#include "common.h"
+
+using namespace idft;
+
+struct idft_mv {
+  idft_ctx* ctx = nullptr;
+  int ncols = 0;
+  DevBuf<double> d;
+};
+
+namespace {
+
+double wall_ms() {
+  using namespace std::chrono;
+  return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+struct AdjointState {
+  bool ready = false;
+  int B = 0;
+  idft_mv* psi = nullptr;
+  idft_mv* x = nullptr;
+  std::vector<double> occ, eig;
+  std::vector<int> pairs;
+  double fermi = 0.0;
+  DevBuf<int> dpairs;
+  DevBuf<double> docc4, dnegEig, uJxW;
+  std::vector<double> Mu;
+};
+
+// Work buffers of one MINRES precision.
+template <typename T>
+struct MinresWork {
+  DevBuf<T> r1, r2, y, v, w1, w, rhs;
+  void ensure(size_t n) {
+    r1.resize(n); r2.resize(n); y.resize(n); v.resize(n); w1.resize(n); w.resize(n); rhs.resize(n);
+  }
+};
+
+}  // namespace
+
+struct idft_ctx {
+  idft_desc d{};
+  Tables t;
+  Geo g{};
+  DevTables dt;
+  hipStream_t s = nullptr;
+  std::string err;
+  int B = 0;
+  int64_t nQ = 0;
+  bool mixed = false;
+  // static per-node / per-quadrature data
+  DevBuf<double> S, sqrtM, invSqrtM, Djac, DjacSqrt, Mdiag, jxw;
+  DevBuf<float> S32;
+  DevBuf<unsigned char> bnd, adjMask;
+  DevBuf<double> potBase, rhoT, weight, veff, rho, u;
+  std::vector<double> hMdiag;
+  // Hamiltonian
+  DevBuf<double> H64;
+  DevBuf<float> H32;
+  bool haveH = false;
+  // eigen state (Loewdin vectors) and work
+  DevBuf<double> X, X2, HX, Xtmp, T2buf, rref, psiAll, force;
+  DevBuf<float> F0, F1, F2, Rf;
+  DevBuf<double> ydev;
+  bool haveHX = false;  // HX == H~ X for the current X and Hamiltonian
+  DevBuf<double> partial, small, small2, occD;
+  std::vector<double> eig, occ, resid;
+  double fermi = 0.0, bup = 0.0, blow = 0.0;
+  bool haveX = false, haveRitz = false, needRR = false;
+  // inverse-problem state
+  int64_t counter = 0;
+  double tolCheb = 1e-8, adjointTol = 1e-11, lossPrev = 0.0, loss = 0.0;
+  bool haveForward = false;
+  std::vector<double> pot0, potSol;
+  idft_stats st{};
+  AdjointState adj;
+  MinresWork<double> mw64;
+  MinresWork<float> mw32;
+  DevBuf<double> mst, mx;
+  DevBuf<int> mflags;
+  idft_mv mvPsi, mvX;
+
+  void sync() { IDFT_HIP(hipStreamSynchronize(s)); }
+  size_t NB(int b) const { return (size_t)t.N * b; }
+
+  // ---------------------------------------------------------------- apply
+  // Y += alpha S H S X (Loewdin Hamiltonian), FP64 or FP32 cell matrices.
+  void happly64(const double* Xv, double* Yv, int b, double alpha) {
+    launch_cell_apply<double>(g, H64.p, S.p, Xv, Yv, b, alpha, s);
+  }
+  void happly32(const float* Xv, float* Yv, int b, double alpha) {
+    if (!H32.p) throw Error(IDFT_ESTATE, "FP32 Hamiltonian not built (precision is FP64)");
+    launch_cell_apply<float>(g, H32.p, S32.p, Xv, Yv, b, alpha, s);
+  }
+  template <typename T>
+  void happly(const T* Xv, T* Yv, int b, double alpha) {
+    if constexpr (std::is_same<T, double>::value) happly64(Xv, Yv, b, alpha);
+    else happly32(Xv, Yv, b, alpha);
+  }
+
+  // ------------------------------------------------------- reductions
+  std::vector<double> coldot(const double* a, const double* b2, int b, const double* w = nullptr) {
+    partial.resize((size_t)1024 * std::max(b, 1));
+    small.resize(std::max(b, 16));
+    const int nb = launch_coldot_partial<double>(t.N, b, a, b2, w, partial.p, s);
+    launch_sum_partials(nb, b, partial.p, small.p, s);
+    std::vector<double> out(b);
+    small.download(out.data(), b, s);
+    sync();
+    return out;
+  }
+  std::vector<double> gram(const double* a, const double* b2, int b) {
+    partial.resize((size_t)gram_partial_blocks(t.N, b) * b * b);
+    small.resize((size_t)b * b);
+    launch_gram(t.N, b, a, b2, partial.p, small.p, s);
+    std::vector<double> G((size_t)b * b);
+    small.download(G.data(), G.size(), s);
+    sync();
+    return G;
+  }
+  void right_mult(const double* a, const std::vector<double>& Q, double* out, int b,
+                  const double* ns = nullptr, double beta = 0.0) {
+    small2.resize(Q.size());
+    small2.upload(Q.data(), Q.size(), s);
+    launch_right_mult(t.N, b, a, small2.p, out, ns, beta, s);
+  }
+  void zero_boundary(double* v, int b) { launch_zero_rows(t.N, b, v, bnd.p, s); }
+
+  // ------------------------------------------------------ Hamiltonian
+  void build_h() {
+    const double hh = 0.5 * t.h;
+    H64.resize((size_t)t.ncell * hamiltonian_tile_elems(t.p));
+    if (mixed) H32.resize((size_t)t.ncell * hamiltonian_tile_elems(t.p));
+    launch_build_hamiltonian(g, dt, hh, veff.p, jxw.p, H64.p, mixed ? H32.p : nullptr, s);
+    haveH = true;
+    haveHX = false;
+  }
+  void set_potential(const double* pot_host) {
+    DevBuf<double> pot;
+    pot.upload(pot_host, (size_t)t.NV, s);
+    veff.resize(nQ);
+    launch_vxc_to_quad(g, dt, pot.p, potBase.p, veff.p, s);
+    build_h();
+    sync();
+  }
+
+  // ------------------------------------------------------------ ChFSI
+  void init_subspace() {
+    std::vector<double> h(NB(B));
+    uint64_t sd = d.seed;
+    std::vector<unsigned char> hb((size_t)t.N);
+    bnd.download(hb.data(), hb.size(), s);
+    sync();
+    for (int64_t n = 0; n < t.N; ++n)
+      for (int j = 0; j < B; ++j) {
+        const double r = rand_pm1(sd);
+        h[(size_t)n * B + j] = hb[n] ? 0.0 : r;
+      }
+    X.upload(h.data(), h.size(), s);
+    haveX = true;
+    haveRitz = false;
+    haveHX = false;
+  }
+
+  template <typename T>
+  void lanczos_t(double& upper, double& lowest) {
+    const int k = std::max(2, d.lanczos_steps);
+    std::vector<double> hv((size_t)t.N), hb;
+    std::vector<unsigned char> mask((size_t)t.N);
+    bnd.download(mask.data(), mask.size(), s);
+    sync();
+    uint64_t sd = d.seed ^ 0x5bd1e995ULL;
+    double nrm = 0.0;
+    for (int64_t n = 0; n < t.N; ++n) {
+      const double r = rand_pm1(sd);
+      hv[n] = mask[n] ? 0.0 : r;
+      nrm += hv[n] * hv[n];
+    }
+    nrm = std::sqrt(nrm);
+    for (auto& v : hv) v /= nrm;
+    DevBuf<double> v64(t.N);
+    v64.upload(hv.data(), hv.size(), s);
+    DevBuf<T> v(t.N), vp(t.N), w(t.N);
+    convert_to(v64.p, v.p, t.N);
+    launch_lincomb<T>(t.N, 1, vp.p, 0.0, nullptr, 0.0, nullptr, nullptr, 0.0, nullptr, nullptr, s);
+    std::vector<double> al, be;
+    double beta = 0.0;
+    for (int j = 0; j < k; ++j) {
+      launch_lincomb<T>(t.N, 1, w.p, 0.0, nullptr, 0.0, nullptr, nullptr, 0.0, nullptr, nullptr, s);
+      happly<T>(v.p, w.p, 1, 1.0);
+      const double a = coldot_t<T>(v.p, w.p, 1)[0];
+      launch_lincomb<T>(t.N, 1, w.p, 1.0, v.p, -a, nullptr, vp.p, -beta, nullptr, nullptr, s);
+      const double b2 = std::sqrt(std::max(coldot_t<T>(w.p, w.p, 1)[0], 0.0));
+      al.push_back(a);
+      be.push_back(b2);
+      beta = b2;
+      if (b2 == 0.0) break;
+      std::swap(vp.p, v.p);
+      launch_lincomb<T>(t.N, 1, v.p, 0.0, w.p, 1.0 / b2, nullptr, nullptr, 0.0, nullptr, nullptr, s);
+    }
+    const int kk = (int)al.size();
+    std::vector<double> T3((size_t)kk * kk, 0.0), ev, V;
+    for (int i = 0; i < kk; ++i) {
+      T3[(size_t)i * kk + i] = al[i];
+      if (i + 1 < kk) T3[(size_t)i * kk + i + 1] = T3[(size_t)(i + 1) * kk + i] = be[i];
+    }
+    sym_eig(kk, T3, ev, V);
+    upper = ev[kk - 1] + std::fabs(be[kk - 1]);
+    lowest = ev[0];
+  }
+
+  template <typename T>
+  std::vector<double> coldot_t(const T* a, const T* b2, int b) {
+    partial.resize((size_t)1024 * std::max(b, 1));
+    small.resize(std::max(b, 16));
+    const int nb = launch_coldot_partial<T>(t.N, b, a, b2, nullptr, partial.p, s);
+    launch_sum_partials(nb, b, partial.p, small.p, s);
+    std::vector<double> out(b);
+    small.download(out.data(), b, s);
+    sync();
+    return out;
+  }
+  void convert_to(const double* a, double* o, int64_t n) {
+    IDFT_HIP(hipMemcpyAsync(o, a, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  }
+  void convert_to(const double* a, float* o, int64_t n) { launch_convert_d2f(n, o, a, s); }
+
+  // Scaled Chebyshev filter (Zhou-Saad) of X in place, FP64 or FP32.
+  template <typename T>
+  void filter_t(T* x0, T* y0, T* z0, int b, int deg, double a, double bb, double a0, T** result) {
+    const double e = 0.5 * (bb - a), c = 0.5 * (bb + a);
+    double sigma = e / (a0 - c);
+    const double sigma1 = sigma, tau = 2.0 / sigma1;
+    T *xk = x0, *yk = y0, *zk = z0;
+    launch_lincomb<T>(t.N, b, yk, 0.0, xk, -c * sigma1 / e, nullptr, nullptr, 0.0, nullptr, nullptr, s);
+    happly<T>(xk, yk, b, sigma1 / e);
+    for (int k = 2; k <= deg; ++k) {
+      const double sigma2 = 1.0 / (tau - sigma);
+      const double f = 2.0 * sigma2 / e;
+      launch_lincomb<T>(t.N, b, zk, 0.0, yk, -f * c, nullptr, xk, -sigma * sigma2, nullptr, nullptr, s);
+      happly<T>(yk, zk, b, f);
+      T* tmp = xk; xk = yk; yk = zk; zk = tmp;
+      sigma = sigma2;
+    }
+    *result = yk;
+  }
+
+  void chebyshev_filter_X(double* Xd, int b, int deg, double a, double bb, double a0, bool fp32) {
+    if (fp32) {
+      F0.resize(NB(b)); F1.resize(NB(b)); F2.resize(NB(b));
+      launch_convert_d2f((int64_t)NB(b), F0.p, Xd, s);
+      float* res = nullptr;
+      filter_t<float>(F0.p, F1.p, F2.p, b, deg, a, bb, a0, &res);
+      launch_convert_f2d((int64_t)NB(b), Xd, res, s);
+    } else {
+      X2.resize(NB(b)); HX.resize(NB(b));
+      double* res = nullptr;
+      Xtmp.resize(NB(b));
+      IDFT_HIP(hipMemcpyAsync(Xtmp.p, Xd, NB(b) * sizeof(double), hipMemcpyDeviceToDevice, s));
+      filter_t<double>(Xtmp.p, X2.p, HX.p, b, deg, a, bb, a0, &res);
+      IDFT_HIP(hipMemcpyAsync(Xd, res, NB(b) * sizeof(double), hipMemcpyDeviceToDevice, s));
+    }
+  }
+
+  // Mixed-precision filter in defect form.  With Ritz pairs (X, L) of the
+  // current Hamiltonian and R = H~X - X L (FP64), D_k = Y_k - X y_k(L) obeys
+  //   D_{k+1} = f (H~ - c) D_k - g D_{k-1} + f R diag(y_k(L)),  D_0 = 0,
+  //   D_1 = (sigma1/e) R,
+  // where y_k are the scalar filter polynomials.  D is carried in FP32 on the
+  // FP32 cell operator; Y_m = X y_m(L) + D_m is formed in FP64, so the FP32
+  // rounding error is proportional to the residual and vanishes at convergence.
+  void chebyshev_defect_filter(int deg, double a, double bb, double a0) {
+    const int b = B;
+    const size_t n = NB(b);
+    F0.resize(n); F1.resize(n); F2.resize(n); Rf.resize(n); X2.resize(n);
+    // R = HX - X diag(eig)
+    small2.resize(b);
+    small2.upload(eig.data(), b, s);
+    launch_lincomb<double>(t.N, b, X2.p, 0.0, HX.p, 1.0, nullptr, X.p, -1.0, small2.p, nullptr, s);
+    launch_convert_d2f((int64_t)n, Rf.p, X2.p, s);
+    const double e = 0.5 * (bb - a), c = 0.5 * (bb + a);
+    double sigma = e / (a0 - c);
+    const double sigma1 = sigma, tau = 2.0 / sigma1;
+    // scalar sequences y_k(lambda_j), k = 0..deg
+    std::vector<double> ys((size_t)(deg + 1) * b);
+    for (int j = 0; j < b; ++j) { ys[j] = 1.0; ys[b + j] = (sigma1 / e) * (eig[j] - c); }
+    {
+      double sg = sigma;
+      for (int k = 1; k < deg; ++k) {
+        const double s2 = 1.0 / (tau - sg), f = 2.0 * s2 / e, gk = sg * s2;
+        for (int j = 0; j < b; ++j)
+          ys[(size_t)(k + 1) * b + j] = f * (eig[j] - c) * ys[(size_t)k * b + j] - gk * ys[(size_t)(k - 1) * b + j];
+        sg = s2;
+      }
+    }
+    ydev.resize(ys.size());
+    ydev.upload(ys.data(), ys.size(), s);
+    float *dm = F0.p, *dk = F1.p, *dn = F2.p;
+    launch_lincomb<float>(t.N, b, dm, 0.0, nullptr, 0.0, nullptr, nullptr, 0.0, nullptr, nullptr, s);
+    launch_lincomb<float>(t.N, b, dk, 0.0, Rf.p, sigma1 / e, nullptr, nullptr, 0.0, nullptr, nullptr, s);
+    for (int k = 1; k < deg; ++k) {
+      const double s2 = 1.0 / (tau - sigma), f = 2.0 * s2 / e, gk = sigma * s2;
+      launch_lincomb3<float>(t.N, b, dn, 0.0, dk, -f * c, nullptr, dm, -gk, nullptr, Rf.p, f,
+                             ydev.p + (size_t)k * b, nullptr, s);
+      happly32(dk, dn, b, f);
+      float* tmp = dm; dm = dk; dk = dn; dn = tmp;
+      sigma = s2;
+    }
+    launch_convert_f2d((int64_t)n, X2.p, dk, s);
+    launch_lincomb<double>(t.N, b, X.p, 0.0, X.p, 1.0, ydev.p + (size_t)deg * b, X2.p, 1.0, nullptr,
+                           nullptr, s);
+  }
+
+  // Two sweeps of Cholesky-QR (SVQB when the Gram matrix is not positive definite).
+  void orthonormalize() {
+    X2.resize(NB(B));
+    for (int pass = 0; pass < 2; ++pass) {
+      std::vector<double> G = gram(X.p, X.p, B), L = G, Q((size_t)B * B, 0.0);
+      if (cholesky_lower(B, L)) {
+        for (int j = 0; j < B; ++j) {
+          std::vector<double> yv(B, 0.0);
+          for (int i = 0; i < B; ++i) {
+            double sv = (i == j) ? 1.0 : 0.0;
+            for (int k = 0; k < i; ++k) sv -= L[(size_t)i * B + k] * yv[k];
+            yv[i] = sv / L[(size_t)i * B + i];
+          }
+          for (int i = 0; i < B; ++i) Q[(size_t)j * B + i] = yv[i];
+        }
+      } else {
+        std::vector<double> lam, V;
+        sym_eig(B, G, lam, V);
+        const double tau = 1e-14 * std::max(lam[B - 1], 1e-300);
+        for (int i = 0; i < B; ++i)
+          for (int j = 0; j < B; ++j)
+            Q[(size_t)i * B + j] = V[(size_t)i * B + j] / std::sqrt(std::max(lam[j], tau));
+      }
+      right_mult(X.p, Q, X2.p, B);
+      std::swap(X.p, X2.p);
+    }
+  }
+
+  void rayleigh_ritz() {
+    HX.resize(NB(B)); X2.resize(NB(B));
+    launch_lincomb<double>(t.N, B, HX.p, 0.0, nullptr, 0.0, nullptr, nullptr, 0.0, nullptr, nullptr, s);
+    happly64(X.p, HX.p, B, 1.0);
+    std::vector<double> Hp = gram(X.p, HX.p, B), Q;
+    for (int i = 0; i < B; ++i)
+      for (int j = i + 1; j < B; ++j) {
+        const double v = 0.5 * (Hp[(size_t)i * B + j] + Hp[(size_t)j * B + i]);
+        Hp[(size_t)i * B + j] = Hp[(size_t)j * B + i] = v;
+      }
+    sym_eig(B, Hp, eig, Q);
+    right_mult(X.p, Q, X2.p, B);
+    std::swap(X.p, X2.p);
+    right_mult(HX.p, Q, X2.p, B);
+    std::swap(HX.p, X2.p);
+    small2.resize(B);
+    small2.upload(eig.data(), B, s);
+    partial.resize((size_t)1024 * B);
+    small.resize(std::max(B, 16));
+    const int nb = launch_residual_partial(t.N, B, HX.p, X.p, small2.p, partial.p, s);
+    launch_sum_partials(nb, B, partial.p, small.p, s);
+    resid.assign(B, 0.0);
+    small.download(resid.data(), B, s);
+    sync();
+    for (auto& r : resid) r = std::sqrt(std::max(r, 0.0));
+    haveRitz = true;
+    haveHX = true;
+  }
+
+  void update_occupancy() {
+    fermi = fermi_level(eig, d.n_electrons, d.temperature);
+    occ.assign(B, 0.0);
+    for (int i = 0; i < B; ++i) occ[i] = fd_occ(eig[i], fermi, d.temperature);
+  }
+
+  void chfsi_pass() {
+    double a0, a;
+    if (haveRitz) { a0 = eig[0]; a = eig[B - 1]; }
+    else { a0 = blow; a = a0 + 0.1 * (bup - a0); }
+    if (mixed && haveRitz) {
+      if (!haveHX) rayleigh_ritz();  // Ritz pairs of the current Hamiltonian
+      a0 = eig[0]; a = eig[B - 1];
+      chebyshev_defect_filter(d.cheb_degree, a, bup, a0);
+    } else {
+      chebyshev_filter_X(X.p, B, d.cheb_degree, a, bup, a0, mixed);
+    }
+    haveHX = false;
+    orthonormalize();
+    rayleigh_ritz();
+    for (double r : resid)
+      if (!std::isfinite(r)) throw Error(IDFT_ENAN, "non-finite Ritz residual");
+  }
+
+  void solve_passes(double tol, int maxPasses, int* passesOut, double* maxResOut) {
+    if (!haveH) throw Error(IDFT_ESTATE, "no Hamiltonian: call set_potential first");
+    if (!haveX) init_subspace();
+    if (mixed) lanczos_t<float>(bup, blow);
+    else lanczos_t<double>(bup, blow);
+    st.spectrum_upper_bound = bup;
+    if (needRR) {
+      orthonormalize();
+      rayleigh_ritz();
+      needRR = false;
+    }
+    int pass = 0;
+    double maxRes = 0.0;
+    do {
+      chfsi_pass();
+      update_occupancy();
+      maxRes = 0.0;
+      int homo = 0;
+      for (int i = 0; i < B; ++i)
+        if (eig[i] < fermi + 1e-3 && homo < i) homo = i;
+      for (int i = 0; i < B; ++i)
+        if (occ[i] > d.frac_occ_tol || i <= homo + d.additional_states)
+          maxRes = std::max(maxRes, resid[i]);
+      pass++;
+    } while (maxRes > tol && pass < maxPasses);
+    st.cheb_passes = pass;
+    st.max_residual = maxRes;
+    st.fermi_energy = fermi;
+    if (passesOut) *passesOut = pass;
+    if (maxResOut) *maxResOut = maxRes;
+  }
+
+  void physical_psi() {
+    psiAll.resize(NB(B));
+    launch_lincomb<double>(t.N, B, psiAll.p, 0.0, X.p, 1.0, nullptr, nullptr, 0.0, nullptr, S.p, s);
+  }
+
+  void density_into(double* out) {
+    occD.resize(B);
+    occD.upload(occ.data(), B, s);
+    launch_density(g, dt, X.p, S.p, B, occD.p, out, s);
+  }
+
+  // ----------------------------------------------------- adjoint problem
+  std::vector<double> pair_quad_sum(const double* A, const double* Bv, int b, const double* wq,
+                                    int64_t wstride) {
+    const int np = (int)adj.pairs.size() / 2;
+    if (np == 0) return {};
+    partial.resize((size_t)t.ncell * np);
+    small.resize(std::max(np, 16));
+    launch_pair_quad(g, dt, A, Bv, b, adj.dpairs.p, np, wq, wstride, partial.p, s);
+    launch_sum_partials((int)t.ncell, np, partial.p, small.p, s);
+    std::vector<double> out(np);
+    small.download(out.data(), np, s);
+    sync();
+    return out;
+  }
+
+  void compute_mu(const std::vector<double>& occv) {
+    const int Bb = adj.B;
+    std::vector<double> v = pair_quad_sum(adj.psi->d.p, adj.psi->d.p, Bb, adj.uJxW.p, t.nq);
+    adj.Mu.assign((size_t)Bb * Bb, 0.0);
+    for (size_t k = 0; k < v.size(); ++k) {
+      const int i = adj.pairs[2 * k], j = adj.pairs[2 * k + 1];
+      adj.Mu[(size_t)i * Bb + j] += 2.0 * occv[i] * v[k];
+    }
+  }
+
+  void update_input_psi(idft_mv* psi, const double* occv, const double* u_dev_or_null,
+                        const double* u_host, const int32_t* pairs, int np, double fermiE,
+                        double* eigv, int b) {
+    if (!psi || psi->ncols != b || b <= 0) throw Error(IDFT_EINVAL, "psi block size mismatch");
+    AdjointState& A = adj;
+    A.B = b; A.psi = psi; A.fermi = fermiE; A.x = nullptr;
+    zero_boundary(psi->d.p, b);
+    A.occ.assign(occv, occv + b);
+    std::vector<double> occ4(b);
+    for (int i = 0; i < b; ++i) occ4[i] = 4.0 * occv[i];
+    A.pairs.assign(pairs, pairs + 2 * np);
+    for (int v : A.pairs)
+      if (v < 0 || v >= b) throw Error(IDFT_EINVAL, "pair index out of range");
+    A.eig.assign(eigv, eigv + b);
+    for (int i = 0; i < b; ++i) eigv[i] = -eigv[i];
+    A.docc4.upload(occ4.data(), b, s);
+    A.dnegEig.upload(eigv, b, s);
+    if (np) A.dpairs.upload(A.pairs.data(), A.pairs.size(), s);
+    A.uJxW.resize(nQ);
+    if (u_dev_or_null) {
+      launch_mul_q(nQ, t.nq, u_dev_or_null, jxw.p, A.uJxW.p, s);
+    } else {
+      DevBuf<double> uq;
+      uq.upload(u_host, nQ, s);
+      launch_mul_q(nQ, t.nq, uq.p, jxw.p, A.uJxW.p, s);
+      sync();
+    }
+    A.ready = true;
+  }
+
+  void compute_rhs(idft_mv* x, double* rhs) {
+    AdjointState& A = adj;
+    if (!A.ready) throw Error(IDFT_ESTATE, "computeRhs before updateInputPsi");
+    const int b = A.B;
+    if (x && x->ncols != b) throw Error(IDFT_EINVAL, "block size mismatch");
+    A.x = x;
+    const double* psi = A.psi->d.p;
+    compute_mu(A.occ);
+    // rhs = -2 sqrt(M) psi Mu   (column i: -2 sum_k Mu[k][i] M^{1/2} psi_k)
+    std::vector<double> C((size_t)b * b);
+    for (size_t k = 0; k < C.size(); ++k) C[k] = -2.0 * A.Mu[k];
+    right_mult(psi, C, rhs, b, sqrtM.p, 0.0);
+    // rhs += 4 f M^{-1/2} R psi, R applied on the fly as N^T diag(u JxW) N
+    T2buf.resize(NB(b));
+    double* T2 = T2buf.p;
+    launch_lincomb<double>(t.N, b, T2, 0.0, nullptr, 0.0, nullptr, nullptr, 0.0, nullptr, nullptr, s);
+    for (int colour = 0; colour < 8; ++colour)
+      launch_quad_operator(g, dt, psi, b, A.uJxW.p, T2, colour, s);
+    zero_boundary(T2, b);
+    launch_lincomb<double>(t.N, b, rhs, 1.0, T2, 1.0, A.docc4.p, nullptr, 0.0, nullptr, invSqrtM.p, s);
+    zero_boundary(rhs, b);
+    sync();
+  }
+
+  void vmult(double* Ax, double* x, int b) {
+    if (!adj.ready) throw Error(IDFT_ESTATE, "vmult before updateInputPsi");
+    zero_boundary(x, b);
+    launch_lincomb<double>(t.N, b, Ax, 0.0, x, 1.0, adj.dnegEig.p, nullptr, 0.0, nullptr, nullptr, s);
+    happly64(x, Ax, b, 1.0);
+  }
+
+  std::vector<double> dot_quad_wise(const double* v1, const double* v2, int b) {
+    return pair_quad_sum(v1, v2, b, jxw.p, 0);
+  }
+
+  void distribute_x() {
+    AdjointState& A = adj;
+    if (!A.ready || !A.x) throw Error(IDFT_ESTATE, "distributeX before computeRhs");
+    const int b = A.B;
+    double* x = A.x->d.p;
+    const double* psi = A.psi->d.p;
+    launch_lincomb<double>(t.N, b, x, 0.0, x, 1.0, nullptr, nullptr, 0.0, nullptr, invSqrtM.p, s);
+    zero_boundary(x, b);
+    const int np = (int)A.pairs.size() / 2;
+    std::vector<double> dot;
+    if (d.gradient_mode == IDFT_GRADIENT_EXACT) {
+      partial.resize((size_t)1024 * std::max(np, 1));
+      small.resize(std::max(np, 16));
+      const int nb = launch_pairdot_partial(t.N, b, x, psi, Mdiag.p, A.dpairs.p, np, partial.p, s);
+      launch_sum_partials(nb, np, partial.p, small.p, s);
+      dot.resize(np);
+      small.download(dot.data(), np, s);
+      sync();
+    } else {
+      dot = dot_quad_wise(x, psi, b);
+    }
+    compute_mu(std::vector<double>(b, 1.0));
+    double sumD = 0.0, etaR = 0.0;
+    for (int i = 0; i < b; ++i) {
+      const double dF = -fd_occ_der(A.eig[i], A.fermi, d.temperature);
+      sumD += dF;
+      etaR += dF * A.Mu[(size_t)i * b + i];
+    }
+    const double eta = std::fabs(etaR) < 1e-16 ? 0.0 : etaR / sumD;
+    std::vector<double> C((size_t)b * b, 0.0);  // x[:, i] += sum_j C[j][i] psi[:, j]
+    for (int k = 0; k < np; ++k) {
+      const int i = A.pairs[2 * k], j = A.pairs[2 * k + 1];
+      const double dFi = fd_occ_der(A.eig[i], A.fermi, d.temperature);
+      double corr = -dFi * A.Mu[(size_t)i * b + j];
+      if (i == j) corr += dFi * eta;
+      C[(size_t)j * b + i] += -dot[k] + corr;
+    }
+    right_mult(psi, C, x, b, nullptr, 1.0);
+    sync();
+  }
+
+  // ------------------------------------------------------------ MINRES
+  template <typename T>
+  MinresWork<T>& work() {
+    if constexpr (std::is_same<T, double>::value) return mw64;
+    else return mw32;
+  }
+
+  // Preconditioned MINRES from x = 0 (x accumulates), per-column tolerance.
+  template <typename T>
+  std::vector<int> minres_core(const T* rhs, double* x, int b, const std::vector<double>& tol,
+                               int maxit) {
+    MinresWork<T>& W = work<T>();
+    W.ensure(NB(b));
+    mst.resize((size_t)MS_COUNT * b);
+    mflags.resize(b);
+    mst.upload(tol.data(), b, s);  // slot 0 temporarily
+    IDFT_HIP(hipMemcpyAsync(mst.p + (size_t)MS_TOL * b, mst.p, b * sizeof(double),
+                            hipMemcpyDeviceToDevice, s));
+    const int64_t N = t.N;
+    launch_lincomb<T>(N, b, W.r1.p, 0.0, rhs, 1.0, nullptr, nullptr, 0.0, nullptr, nullptr, s);
+    launch_lincomb<T>(N, b, W.r2.p, 0.0, rhs, 1.0, nullptr, nullptr, 0.0, nullptr, nullptr, s);
+    launch_lincomb<T>(N, b, W.y.p, 0.0, rhs, 1.0, nullptr, nullptr, 0.0, nullptr, Djac.p, s);
+    launch_lincomb<T>(N, b, W.w1.p, 0.0, nullptr, 0.0, nullptr, nullptr, 0.0, nullptr, nullptr, s);
+    launch_lincomb<T>(N, b, W.w.p, 0.0, nullptr, 0.0, nullptr, nullptr, 0.0, nullptr, nullptr, s);
+    partial.resize((size_t)1024 * b);
+    int nb = launch_coldot_partial<T>(N, b, W.r1.p, W.y.p, nullptr, partial.p, s);
+    launch_minres_start(b, nb, partial.p, mst.p, s);
+    const int check = std::max(1, d.check_every);
+    std::vector<int> flags(b);
+    T *r1 = W.r1.p, *r2 = W.r2.p, *w1 = W.w1.p, *w = W.w.p;
+    for (int k = 0; k < maxit; ++k) {
+      if (k % check == 0) {
+        launch_copy_done(b, mst.p, mflags.p, s);
+        IDFT_HIP(hipMemcpyAsync(flags.data(), mflags.p, b * sizeof(int), hipMemcpyDeviceToHost, s));
+        sync();
+        bool all = true;
+        for (int f : flags) all = all && f;
+        if (all) break;
+      }
+      launch_lincomb<T>(N, b, W.v.p, 0.0, W.y.p, 1.0, mst.p + (size_t)MS_CV * b, nullptr, 0.0,
+                        nullptr, nullptr, s);
+      launch_lincomb<T>(N, b, W.y.p, 0.0, W.v.p, 1.0, adj.dnegEig.p, r1, 1.0,
+                        mst.p + (size_t)MS_CR1 * b, nullptr, s);
+      happly<T>(W.v.p, W.y.p, b, 1.0);
+      nb = launch_coldot_partial<T>(N, b, W.v.p, W.y.p, nullptr, partial.p, s);
+      launch_minres_alpha(b, nb, partial.p, mst.p, s);
+      nb = launch_minres_lanczos<T>(N, b, W.y.p, r2, r1, Djac.p, mst.p, partial.p, s);
+      launch_minres_givens(b, nb, partial.p, mst.p, s);
+      launch_minres_update<T>(N, b, W.v.p, w1, w, x, mst.p, s);
+      std::swap(r1, r2);
+      std::swap(w1, w);
+    }
+    std::vector<double> its(b);
+    IDFT_HIP(hipMemcpyAsync(its.data(), mst.p + (size_t)MS_IT * b, b * sizeof(double),
+                            hipMemcpyDeviceToHost, s));
+    sync();
+    std::vector<int> out(b);
+    for (int j = 0; j < b; ++j) out[j] = (int)its[j];
+    return out;
+  }
+
+  void minres(idft_mv* x, double tol, int maxit, int distribute, int32_t* iters) {
+    AdjointState& A = adj;
+    if (!A.ready) throw Error(IDFT_ESTATE, "no adjoint problem");
+    const int b = A.B;
+    if (x->ncols != b) throw Error(IDFT_EINVAL, "block size mismatch");
+    DevBuf<double>& rhs = mw64.rhs;
+    rhs.resize(NB(b));
+    compute_rhs(x, rhs.p);
+    {
+      std::vector<double> nr = coldot(rhs.p, rhs.p, b);
+      for (double v : nr)
+        if (!std::isfinite(v)) throw Error(IDFT_ENAN, "non-finite adjoint rhs");
+    }
+    double* xp = x->d.p;
+    launch_lincomb<double>(t.N, b, xp, 0.0, nullptr, 0.0, nullptr, nullptr, 0.0, nullptr, nullptr, s);
+    std::vector<int> total(b, 0);
+    if (!mixed) {
+      total = minres_core<double>(rhs.p, xp, b, std::vector<double>(b, tol), maxit);
+    } else {
+      // FP32 inner MINRES on the FP32 cell operator, FP64 iterative refinement:
+      // r = b - (H~ - eps) x in FP64, inner solve to rel. 1e-5, x += delta.
+      rref.resize(NB(b));
+      double* r = rref.p;
+      mw32.ensure(NB(b));
+      int used = 0, sweeps = 0;
+      while (used < maxit) {
+        launch_lincomb<double>(t.N, b, r, 0.0, xp, -1.0, adj.dnegEig.p, rhs.p, 1.0, nullptr, nullptr, s);
+        happly64(xp, r, b, -1.0);
+        std::vector<double> rn = coldot(r, r, b, Djac.p);
+        std::vector<double> tl(b);
+        bool all = true;
+        for (int j = 0; j < b; ++j) {
+          const double nrm = std::sqrt(std::max(rn[j], 0.0));
+          if (!std::isfinite(nrm)) throw Error(IDFT_ENAN, "non-finite refinement residual");
+          if (nrm <= tol) tl[j] = INFINITY;  // converged column: start as done
+          else { tl[j] = std::max(tol, 1e-5 * nrm); all = false; }
+        }
+        if (all) break;
+        launch_convert_d2f((int64_t)NB(b), mw32.rhs.p, r, s);
+        std::vector<int> it = minres_core<float>(mw32.rhs.p, xp, b, tl, maxit - used);
+        int mx = 0;
+        for (int j = 0; j < b; ++j) { total[j] += it[j]; mx = std::max(mx, it[j]); }
+        used += std::max(mx, 1);
+        sweeps++;
+      }
+      st.refinements += sweeps;
+    }
+    int mx = 0;
+    int64_t tot = 0;
+    for (int j = 0; j < b; ++j) {
+      mx = std::max(mx, total[j]);
+      tot += total[j];
+      if (iters) iters[j] = total[j];
+    }
+    st.minres_iters_max = std::max(st.minres_iters_max, mx);
+    st.minres_iters_total += tot;
+    if (distribute) distribute_x();
+    sync();
+  }
+
+  // ------------------------------------------------------ getForceVector
+  void forward(const double* pot, double* lossOut) {
+    const double t0 = wall_ms();
+    if (counter > 3) {
+      const double prev = tolCheb;
+      tolCheb = std::min(d.cheb_tol, lossPrev / d.adaptive_factor_cheb);
+      tolCheb = std::min(tolCheb, prev);
+    } else {
+      tolCheb = std::min(d.cheb_tol, d.initial_tol_cheb);
+    }
+    set_potential(pot);
+    solve_passes(tolCheb, d.max_cheb_passes, nullptr, nullptr);
+    st.cheb_tol_used = tolCheb;
+    sync();
+    const double t1 = wall_ms();
+    rho.resize(nQ); u.resize(nQ);
+    density_into(rho.p);
+    partial.resize((size_t)kLossBlocks * 4);
+    small.resize(16);
+    launch_loss(nQ, t.nq, rho.p, rhoT.p, weight.p, veff.p, jxw.p, u.p, partial.p, s);
+    launch_sum_partials(kLossBlocks, 4, partial.p, small.p, s);
+    double v4[4];
+    small.download(v4, 4, s);
+    sync();
+    loss = v4[0];
+    st.loss_unweighted = v4[1];
+    st.l1_density_error = v4[2];
+    st.int_rho_veff = v4[3];
+    st.t_eigen_ms = t1 - t0;
+    st.t_density_ms = wall_ms() - t1;
+    if (lossOut) *lossOut = loss;
+    haveForward = true;
+  }
+
+  void backward(double* forceOut) {
+    if (!haveForward) throw Error(IDFT_ESTATE, "backward before forward");
+    const double t0 = wall_ms();
+    physical_psi();
+    force.resize(t.NV);
+    launch_lincomb<double>(t.NV, 1, force.p, 0.0, nullptr, 0.0, nullptr, nullptr, 0.0, nullptr, nullptr, s);
+    st.minres_iters_max = 0;
+    st.minres_iters_total = 0;
+    st.refinements = 0;
+    double tForce = 0.0;
+    const int defBS = d.block_size > 0 ? d.block_size : B;
+    int jvec = 0;
+    while (jvec < B) {
+      int bs = std::min(defBS, B - jvec);
+      while (true) {
+        const int last = jvec + bs - 1;
+        if (last + 1 != B && std::fabs(eig[last] - eig[last + 1]) < d.degeneracy_tol && bs > 1) bs--;
+        else break;
+      }
+      mvPsi.ncols = bs; mvPsi.ctx = this;
+      mvPsi.d.resize(NB(bs));
+      launch_copy_cols(t.N, B, jvec, bs, psiAll.p, mvPsi.d.p, s);
+      std::vector<double> occb(bs), shifts(bs);
+      for (int j = 0; j < bs; ++j) { shifts[j] = eig[jvec + j]; occb[j] = occ[jvec + j]; }
+      // degeneracy map (InverseDFTSolverFunction.cpp:37-77)
+      std::map<int, std::set<int>> groups;
+      std::map<int, int> toGroup;
+      int count = 0;
+      for (int i = 0; i < bs; ++i) {
+        auto it = toGroup.find(i);
+        int gi;
+        if (it != toGroup.end()) gi = it->second;
+        else { gi = count++; groups[gi].insert(i); toGroup[i] = gi; }
+        for (int j = 0; j < bs; ++j)
+          if (std::fabs(shifts[i] - shifts[j]) < d.degeneracy_tol) { groups[gi].insert(j); toGroup[j] = gi; }
+      }
+      std::vector<int32_t> pairs;
+      for (int i = 0; i < bs; ++i)
+        for (int j : groups[toGroup[i]]) { pairs.push_back(i); pairs.push_back(j); }
+      update_input_psi(&mvPsi, occb.data(), u.p, nullptr, pairs.data(), (int)pairs.size() / 2,
+                       fermi, shifts.data(), bs);
+      adjointTol = std::min(adjointTol, tolCheb / d.adaptive_factor_adjoint);
+      st.adjoint_tol_used = adjointTol;
+      mvX.ncols = bs; mvX.ctx = this;
+      mvX.d.resize(NB(bs));
+      minres(&mvX, adjointTol, d.adjoint_max_iter, 1, nullptr);
+      launch_zero_rows(t.N, bs, mvX.d.p, adjMask.p, s);
+      const double tf = wall_ms();
+      const int mode = d.gradient_mode == IDFT_GRADIENT_REFERENCE ? 0 : 1;
+      const double jc = std::pow(0.25 * t.h, 3);
+      for (int colour = 0; colour < 8; ++colour)
+        launch_force(g, dt, mvPsi.d.p, mvX.d.p, bs, mode, jc, jxw.p, force.p, colour, s);
+      sync();
+      tForce += wall_ms() - tf;
+      jvec += bs;
+    }
+    force.download(forceOut, t.NV, s);
+    sync();
+    st.t_adjoint_ms = wall_ms() - t0 - tForce;
+    st.t_force_ms = tForce;
+    lossPrev = loss;
+    counter++;
+    st.n_evaluations = counter;
+    haveForward = false;
+  }
+};
+
+
+// ======================================================================= ABI
+namespace {
+
+template <typename F>
+int guarded(idft_ctx* c, F&& f) {
+  try {
+    f();
+    return IDFT_OK;
+  } catch (const Error& e) {
+    if (c) c->err = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    if (c) c->err = e.what();
+    return IDFT_EINVAL;
+  }
+}
+
+idft_ctx* create_ctx(const idft_desc* d, const double* pot_base_q, const double* rho_target_q,
+                     const double* weight_q, const double* rho_target_nodal) {
+  idft_ctx* c = new idft_ctx();
+  try {
+    c->d = *d;
+    c->t.init(*d);
+    const Tables& t = c->t;
+    c->g = Geo{t.nc, t.p, t.n1, t.N1, t.q1, t.NV1, t.N, t.ncell};
+    c->B = d->n_eig;
+    c->nQ = t.ncell * (int64_t)t.nq;
+    c->mixed = d->precision == IDFT_MIXED;
+    IDFT_HIP(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
+    hipStream_t s = c->s;
+    c->dt.Lq.upload(t.Lq.data(), t.Lq.size(), s);
+    c->dt.Lc.upload(t.Lc.data(), t.Lc.size(), s);
+    c->dt.M1.upload(t.M1.data(), t.M1.size(), s);
+    c->dt.D1.upload(t.D1.data(), t.D1.size(), s);
+    c->dt.Wq.upload(t.Wq.data(), t.Wq.size(), s);
+    c->dt.Wc.upload(t.Wc.data(), t.Wc.size(), s);
+    // uniform-cell JxW at the parent Gauss points
+    const double hh = 0.5 * t.h, J = hh * hh * hh;
+    std::vector<double> jw(t.nq);
+    for (int qz = 0; qz < t.q1; ++qz)
+      for (int qy = 0; qy < t.q1; ++qy)
+        for (int qx = 0; qx < t.q1; ++qx)
+          jw[(qz * t.q1 + qy) * t.q1 + qx] = J * t.gq_w[qx] * t.gq_w[qy] * t.gq_w[qz];
+    c->jxw.upload(jw.data(), jw.size(), s);
+    // lumped mass and stiffness diagonal
+    std::vector<double> M(t.N, 0.0), K(t.N, 0.0);
+    const int n1 = t.n1;
+    std::vector<double> Kloc(t.ne);
+    for (int cc = 0; cc < n1; ++cc)
+      for (int b = 0; b < n1; ++b)
+        for (int a = 0; a < n1; ++a)
+          Kloc[(cc * n1 + b) * n1 + a] =
+              hh * (t.D1[a * n1 + a] * t.M1[b * n1 + b] * t.M1[cc * n1 + cc] +
+                    t.M1[a * n1 + a] * t.D1[b * n1 + b] * t.M1[cc * n1 + cc] +
+                    t.M1[a * n1 + a] * t.M1[b * n1 + b] * t.D1[cc * n1 + cc]);
+    for (int64_t e = 0; e < t.ncell; ++e) {
+      const int64_t cx = e % t.nc, cy = (e / t.nc) % t.nc, cz = e / ((int64_t)t.nc * t.nc);
+      for (int cc = 0; cc < n1; ++cc)
+        for (int b = 0; b < n1; ++b)
+          for (int a = 0; a < n1; ++a) {
+            const int64_t n = ((cz * t.p + cc) * t.N1 + (cy * t.p + b)) * (int64_t)t.N1 + (cx * t.p + a);
+            M[n] += J * t.gll_w[a] * t.gll_w[b] * t.gll_w[cc];
+            K[n] += Kloc[(cc * n1 + b) * n1 + a];
+          }
+    }
+    std::vector<double> Sv(t.N), sq(t.N), isq(t.N), Dj(t.N), Djs(t.N);
+    std::vector<float> S32(t.N);
+    std::vector<unsigned char> bnd(t.N), am(t.N);
+    for (int64_t n = 0; n < t.N; ++n) {
+      const int64_t i = n % t.N1, j = (n / t.N1) % t.N1, k = n / ((int64_t)t.N1 * t.N1);
+      bnd[n] = (i == 0 || j == 0 || k == 0 || i == t.N1 - 1 || j == t.N1 - 1 || k == t.N1 - 1);
+      am[n] = rho_target_nodal[n] < d->rho_tol_constraints;
+      sq[n] = std::sqrt(M[n]);
+      isq[n] = 1.0 / sq[n];
+      Sv[n] = bnd[n] ? 0.0 : isq[n];
+      S32[n] = (float)Sv[n];
+      Dj[n] = 2.0 * M[n] / K[n];
+      Djs[n] = std::sqrt(Dj[n]);
+    }
+    c->hMdiag = M;
+    c->Mdiag.upload(M.data(), t.N, s);
+    c->S.upload(Sv.data(), t.N, s);
+    c->S32.upload(S32.data(), t.N, s);
+    c->sqrtM.upload(sq.data(), t.N, s);
+    c->invSqrtM.upload(isq.data(), t.N, s);
+    c->Djac.upload(Dj.data(), t.N, s);
+    c->DjacSqrt.upload(Djs.data(), t.N, s);
+    c->bnd.upload(bnd.data(), t.N, s);
+    c->adjMask.upload(am.data(), t.N, s);
+    c->potBase.upload(pot_base_q, c->nQ, s);
+    c->rhoT.upload(rho_target_q, c->nQ, s);
+    c->weight.upload(weight_q, c->nQ, s);
+    c->adjointTol = d->adjoint_initial_tol;
+    c->pot0.assign(t.NV, 0.0);
+    c->potSol.assign(t.NV, 0.0);
+    c->sync();
+  } catch (...) {
+    if (c->s) (void)hipStreamDestroy(c->s);
+    delete c;
+    throw;
+  }
+  return c;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* idft_backend(void) { return "hip-gfx950"; }
+
+int idft_hip_init(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n < 1) return IDFT_EHIP;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, 0) != hipSuccess) return IDFT_EHIP;
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return IDFT_EINVAL;
+  return IDFT_OK;
+}
+
+int idft_create(const idft_desc* d, const double* pot_base_q, const double* rho_target_q,
+                const double* weight_q, const double* rho_target_nodal, idft_ctx** out) {
+  if (!out) return IDFT_EINVAL;
+  *out = nullptr;
+  if (!desc_valid(d) || !pot_base_q || !rho_target_q || !weight_q || !rho_target_nodal)
+    return IDFT_EINVAL;
+  if (d->precision != IDFT_FP64 && d->precision != IDFT_MIXED) return IDFT_EINVAL;
+  try {
+    *out = create_ctx(d, pot_base_q, rho_target_q, weight_q, rho_target_nodal);
+  } catch (const Error& e) {
+    return e.code;
+  } catch (...) {
+    return IDFT_EHIP;
+  }
+  return IDFT_OK;
+}
+
+void idft_destroy(idft_ctx* c) {
+  if (!c) return;
+  if (c->s) (void)hipStreamSynchronize(c->s);
+  hipStream_t s = c->s;
+  delete c;
+  if (s) (void)hipStreamDestroy(s);
+}
+
+const char* idft_last_error(const idft_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int idft_get_sizes(const idft_ctx* c, idft_sizes* s) {
+  if (!c || !s) return IDFT_EINVAL;
+  return idft_geometry_sizes(&c->d, s);
+}
+
+int idft_get_stats(const idft_ctx* c, idft_stats* s) {
+  if (!c || !s) return IDFT_EINVAL;
+  *s = c->st;
+  return IDFT_OK;
+}
+
+int idft_set_potential(idft_ctx* c, const double* pot) {
+  if (!c || !pot) return IDFT_EINVAL;
+  return guarded(c, [&] { c->set_potential(pot); });
+}
+
+int idft_set_veff_quad(idft_ctx* c, const double* vq) {
+  if (!c || !vq) return IDFT_EINVAL;
+  return guarded(c, [&] {
+    c->veff.upload(vq, c->nQ, c->s);
+    c->build_h();
+    c->sync();
+  });
+}
+
+int idft_eigensolve(idft_ctx* c, double tol, int32_t maxp, double* ev, double* mr, int32_t* passes) {
+  if (!c) return IDFT_EINVAL;
+  return guarded(c, [&] {
+    int p = 0;
+    c->solve_passes(tol, maxp, &p, mr);
+    if (passes) *passes = p;
+    if (ev) std::copy(c->eig.begin(), c->eig.end(), ev);
+  });
+}
+
+int idft_get_eigenvectors(const idft_ctx* cc, double* psi) {
+  idft_ctx* c = const_cast<idft_ctx*>(cc);
+  if (!c || !psi) return IDFT_EINVAL;
+  if (!c->haveX) return IDFT_ESTATE;
+  return guarded(c, [&] {
+    c->physical_psi();
+    c->psiAll.download(psi, c->NB(c->B), c->s);
+    c->sync();
+  });
+}
+
+int idft_set_eigenvectors(idft_ctx* c, const double* psi) {
+  if (!c || !psi) return IDFT_EINVAL;
+  return guarded(c, [&] {
+    c->X.upload(psi, c->NB(c->B), c->s);
+    launch_lincomb<double>(c->t.N, c->B, c->X.p, 0.0, c->X.p, 1.0, nullptr, nullptr, 0.0, nullptr,
+                           c->sqrtM.p, c->s);
+    c->zero_boundary(c->X.p, c->B);
+    c->sync();
+    c->haveX = true;
+    c->haveRitz = false;
+    c->haveHX = false;
+    c->needRR = true;
+  });
+}
+
+int idft_get_occupancies(const idft_ctx* c, double* occ, double* fermi) {
+  if (!c) return IDFT_EINVAL;
+  if (c->occ.empty()) return IDFT_ESTATE;
+  if (occ) std::copy(c->occ.begin(), c->occ.end(), occ);
+  if (fermi) *fermi = c->fermi;
+  return IDFT_OK;
+}
+
+int idft_get_density(idft_ctx* c, double* rho) {
+  if (!c || !rho) return IDFT_EINVAL;
+  if (!c->haveX || c->occ.empty()) {
+    c->err = "no eigenvectors/occupancies";
+    return IDFT_ESTATE;
+  }
+  return guarded(c, [&] {
+    DevBuf<double> r(c->nQ);
+    c->density_into(r.p);
+    r.download(rho, c->nQ, c->s);
+    c->sync();
+  });
+}
+
+int idft_apply_hamiltonian(idft_ctx* c, const double* x, double* y, int32_t ncols, int32_t prec) {
+  if (!c || !x || !y || ncols < 1 || ncols > kMaxCols) return IDFT_EINVAL;
+  if (!c->haveH) { c->err = "no Hamiltonian"; return IDFT_ESTATE; }
+  return guarded(c, [&] {
+    const size_t n = c->NB(ncols);
+    DevBuf<double> xd, yd(n);
+    xd.upload(x, n, c->s);
+    if (prec == IDFT_MIXED) {
+      DevBuf<float> xf(n), yf(n);
+      launch_convert_d2f((int64_t)n, xf.p, xd.p, c->s);
+      launch_lincomb<float>(c->t.N, ncols, yf.p, 0.0, nullptr, 0.0, nullptr, nullptr, 0.0, nullptr, nullptr, c->s);
+      c->happly32(xf.p, yf.p, ncols, 1.0);
+      launch_convert_f2d((int64_t)n, yd.p, yf.p, c->s);
+    } else {
+      launch_lincomb<double>(c->t.N, ncols, yd.p, 0.0, nullptr, 0.0, nullptr, nullptr, 0.0, nullptr, nullptr, c->s);
+      c->happly64(xd.p, yd.p, ncols, 1.0);
+    }
+    yd.download(y, n, c->s);
+    c->sync();
+  });
+}
+
+int idft_chebyshev_filter(idft_ctx* c, double* x, int32_t ncols, int32_t deg, double a, double b,
+                          double a0, int32_t prec) {
+  if (!c || !x || ncols < 1 || ncols > kMaxCols || deg < 1) return IDFT_EINVAL;
+  if (!c->haveH) { c->err = "no Hamiltonian"; return IDFT_ESTATE; }
+  return guarded(c, [&] {
+    DevBuf<double> xd;
+    xd.upload(x, c->NB(ncols), c->s);
+    c->chebyshev_filter_X(xd.p, ncols, deg, a, b, a0, prec == IDFT_MIXED);
+    xd.download(x, c->NB(ncols), c->s);
+    c->sync();
+  });
+}
+
+int idft_mv_create(idft_ctx* c, int32_t ncols, idft_mv** out) {
+  if (!c || !out || ncols < 1 || ncols > kMaxCols) return IDFT_EINVAL;
+  return guarded(c, [&] {
+    idft_mv* v = new idft_mv();
+    v->ctx = c;
+    v->ncols = ncols;
+    v->d.resize(c->NB(ncols));
+    launch_lincomb<double>(c->t.N, ncols, v->d.p, 0.0, nullptr, 0.0, nullptr, nullptr, 0.0, nullptr,
+                           nullptr, c->s);
+    c->sync();
+    *out = v;
+  });
+}
+
+void idft_mv_destroy(idft_mv* v) { delete v; }
+
+int idft_mv_set(idft_ctx* c, idft_mv* v, const double* h) {
+  if (!c || !v || !h) return IDFT_EINVAL;
+  return guarded(c, [&] {
+    v->d.upload(h, c->NB(v->ncols), c->s);
+    c->sync();
+  });
+}
+
+int idft_mv_get(idft_ctx* c, const idft_mv* v, double* h) {
+  if (!c || !v || !h) return IDFT_EINVAL;
+  return guarded(c, [&] {
+    v->d.download(h, c->NB(v->ncols), c->s);
+    c->sync();
+  });
+}
+
+int idft_adj_update_input_psi(idft_ctx* c, idft_mv* psi, const double* occ, const double* u_q,
+                              const int32_t* pairs, int32_t np, double fermi, double* eigv,
+                              int32_t block) {
+  if (!c || !psi || !occ || !u_q || (!pairs && np) || !eigv || np < 0) return IDFT_EINVAL;
+  return guarded(c, [&] {
+    c->update_input_psi(psi, occ, nullptr, u_q, pairs, np, fermi, eigv, block);
+    c->sync();
+  });
+}
+
+int idft_adj_compute_rhs(idft_ctx* c, idft_mv* ndbc, idft_mv* x, idft_mv* rhs) {
+  if (!c || !x || !rhs) return IDFT_EINVAL;
+  (void)ndbc;
+  return guarded(c, [&] {
+    if (rhs->ncols != c->adj.B) throw Error(IDFT_EINVAL, "block size mismatch");
+    c->compute_rhs(x, rhs->d.p);
+  });
+}
+
+int idft_adj_vmult(idft_ctx* c, idft_mv* Ax, idft_mv* x, int32_t block) {
+  if (!c || !Ax || !x) return IDFT_EINVAL;
+  return guarded(c, [&] {
+    if (block != c->adj.B || x->ncols != block || Ax->ncols != block)
+      throw Error(IDFT_EINVAL, "block size mismatch");
+    c->vmult(Ax->d.p, x->d.p, block);
+    c->sync();
+  });
+}
+
+int idft_adj_precondition_jacobi(idft_ctx* c, idft_mv* dst, const idft_mv* src, double) {
+  if (!c || !dst || !src || dst->ncols != src->ncols) return IDFT_EINVAL;
+  return guarded(c, [&] {
+    launch_lincomb<double>(c->t.N, src->ncols, dst->d.p, 0.0, src->d.p, 1.0, nullptr, nullptr, 0.0,
+                           nullptr, c->Djac.p, c->s);
+    c->sync();
+  });
+}
+
+int idft_adj_precondition_jacobi_sqrt(idft_ctx* c, idft_mv* dst, const idft_mv* src, double) {
+  if (!c || !dst || !src || dst->ncols != src->ncols) return IDFT_EINVAL;
+  return guarded(c, [&] {
+    launch_lincomb<double>(c->t.N, src->ncols, dst->d.p, 0.0, src->d.p, 1.0, nullptr, nullptr, 0.0,
+                           nullptr, c->DjacSqrt.p, c->s);
+    c->sync();
+  });
+}
+
+int idft_adj_distribute_x(idft_ctx* c) {
+  if (!c) return IDFT_EINVAL;
+  return guarded(c, [&] { c->distribute_x(); });
+}
+
+int idft_adj_dot_quad_wise(idft_ctx* c, idft_mv* v1, idft_mv* v2, double* out) {
+  if (!c || !v1 || !v2 || !out) return IDFT_EINVAL;
+  return guarded(c, [&] {
+    if (!c->adj.ready) throw Error(IDFT_ESTATE, "no adjoint problem");
+    std::vector<double> o = c->dot_quad_wise(v1->d.p, v2->d.p, c->adj.B);
+    std::copy(o.begin(), o.end(), out);
+  });
+}
+
+int idft_minres_solve(idft_ctx* c, idft_mv* x, idft_mv* ndbc, double tol, int32_t maxit,
+                      int32_t distribute, int32_t* iters) {
+  if (!c || !x || maxit < 0) return IDFT_EINVAL;
+  (void)ndbc;
+  return guarded(c, [&] {
+    c->st.minres_iters_max = 0;
+    c->st.minres_iters_total = 0;
+    c->st.refinements = 0;
+    c->minres(x, tol, maxit, distribute, iters);
+  });
+}
+
+int idft_forward(idft_ctx* c, const double* pot, double* loss) {
+  if (!c || !pot) return IDFT_EINVAL;
+  return guarded(c, [&] { c->forward(pot, loss); });
+}
+
+int idft_backward(idft_ctx* c, double* force) {
+  if (!c || !force) return IDFT_EINVAL;
+  return guarded(c, [&] { c->backward(force); });
+}
+
+int idft_get_force(idft_ctx* c, double* pot, double* force, double* loss) {
+  if (!c || !pot || !force || !loss) return IDFT_EINVAL;
+  return guarded(c, [&] {
+    const double t0 = wall_ms();
+    c->forward(pot, loss);
+    c->backward(force);
+    c->st.t_total_ms = wall_ms() - t0;
+  });
+}
+
+int idft_dot(idft_ctx* c, const double* v1, const double* v2, int64_t n, int32_t block, double* out) {
+  if (!c || !v1 || !v2 || !out || block < 1) return IDFT_EINVAL;
+  for (int i = 0; i < block; ++i) out[i] = 0.0;
+  for (int64_t i = 0; i < n; ++i) out[i % block] += v1[i] * v2[i];
+  return IDFT_OK;
+}
+
+int idft_set_initial_guess(idft_ctx* c, const double* pot) {
+  if (!c || !pot) return IDFT_EINVAL;
+  c->pot0.assign(pot, pot + c->t.NV);
+  return IDFT_OK;
+}
+int idft_get_initial_guess(const idft_ctx* c, double* pot) {
+  if (!c || !pot) return IDFT_EINVAL;
+  std::copy(c->pot0.begin(), c->pot0.end(), pot);
+  return IDFT_OK;
+}
+int idft_set_solution(idft_ctx* c, const double* pot) {
+  if (!c || !pot) return IDFT_EINVAL;
+  c->potSol.assign(pot, pot + c->t.NV);
+  return IDFT_OK;
+}
+int idft_get_solution(const idft_ctx* c, double* pot) {
+  if (!c || !pot) return IDFT_EINVAL;
+  std::copy(c->potSol.begin(), c->potSol.end(), pot);
+  return IDFT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,1207 @@
+// Device kernels of the MI355X inverse-DFT hot path (gfx950 / CDNA4).
+//
+// Layouts
+//   multivectors  node-major X[node * B + col]
+//   quad data     [cell][q], q = (qz*Q + qy)*Q + qx
+//   H tiles       per cell NEP x NEP (NEP = ne rounded up to 16), stored as
+//                 [rt][kg][lane][s] so that lane l of a wave holds the A operand
+//                 of v_mfma_*_16x16x4 for four consecutive k-steps:
+//                 H[rt*16 + l%16][kg*16 + s*4 + l/16],  s = 0..3.
+#include <hip/hip_runtime.h>
+
+#include "common.h"
+
+namespace idft {
+
+namespace {
+
+constexpr int kThreads = 256;
+
+__host__ __device__ constexpr int ne_of(int p) { return (p + 1) * (p + 1) * (p + 1); }
+__host__ __device__ constexpr int nep_of(int p) { return (ne_of(p) + 15) / 16 * 16; }
+
+__device__ inline int64_t cell_node(const Geo& g, int64_t cx, int64_t cy, int64_t cz, int l) {
+  const int a = l % g.n1, b = (l / g.n1) % g.n1, c = l / (g.n1 * g.n1);
+  return ((cz * g.p + c) * g.N1 + (cy * g.p + b)) * (int64_t)g.N1 + (cx * g.p + a);
+}
+
+// Cell of colour `colour` handled by work-group w.  The 8 XCDs receive
+// work-groups round-robin; each XCD is given a contiguous run of cells so its
+// L2 sees spatially adjacent cells.
+// The grid has 8 * ceil(nw / 8) work-groups, so (xcd, slot) covers every k.
+__device__ inline int64_t colour_cell(const Geo& g, int colour, int64_t w, int64_t nw) {
+  const int64_t per = (nw + 7) / 8;
+  const int64_t xcd = w % 8, slot = w / 8;
+  const int64_t k = xcd * per + slot;  // position inside the colour
+  if (k >= nw) return -1;
+  const int64_t hc = (g.nc + 1 - (colour & 1)) / 2;
+  const int64_t hy = (g.nc + 1 - ((colour >> 1) & 1)) / 2;
+  const int64_t kx = k % hc, ky = (k / hc) % hy, kz = k / (hc * hy);
+  const int64_t cx = 2 * kx + (colour & 1), cy = 2 * ky + ((colour >> 1) & 1),
+                cz = 2 * kz + ((colour >> 2) & 1);
+  return (cz * g.nc + cy) * g.nc + cx;
+}
+
+inline unsigned colour_grid(int64_t nw) { return (unsigned)(8 * ((nw + 7) / 8)); }
+
+inline int64_t colour_count(const Geo& g, int colour) {
+  const int64_t hx = (g.nc + 1 - (colour & 1)) / 2;
+  const int64_t hy = (g.nc + 1 - ((colour >> 1) & 1)) / 2;
+  const int64_t hz = (g.nc + 1 - ((colour >> 2) & 1)) / 2;
+  return hx * hy * hz;
+}
+
+// ------------------------------------------------------------ MFMA apply
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+template <typename T> struct Mfma;
+// Output rows held by lane group lk (= lane / 16), item i, of the 16x16x4 MFMA:
+// f32 keeps 4 consecutive rows per group, f64 interleaves them (measured on gfx950).
+template <> struct Mfma<float> {
+  typedef v4f acc_t;
+  static __device__ inline int row(int lk, int i) { return 4 * lk + i; }
+  static __device__ inline acc_t run(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+template <> struct Mfma<double> {
+  typedef v4d acc_t;
+  static __device__ inline int row(int lk, int i) { return lk + 4 * i; }
+  static __device__ inline acc_t run(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+
+template <typename T> struct Vec4;
+template <> struct Vec4<float> { typedef float4 type; };
+template <> struct Vec4<double> { typedef double4 type; };
+
+__host__ __device__ constexpr int ldb_of(int bt) { return (bt % 32 == 16) ? bt : bt + 16; }
+
+// One work-group = one cell of the given colour; 4 waves share the row tiles.
+template <typename T, int P, int NCT>
+__global__ __launch_bounds__(kThreads) void k_cell_apply(Geo g, const T* __restrict__ H,
+                                                         const T* __restrict__ S,
+                                                         const T* __restrict__ X,
+                                                         T* __restrict__ Y, int B, T alpha,
+                                                         int colour, int64_t nw) {
+  constexpr int NE = ne_of(P), NEP = nep_of(P), RT = NEP / 16, BT = NCT * 16;
+  constexpr int LDB = ldb_of(BT);
+  __shared__ T xs[NEP * LDB];
+  __shared__ T sc[NEP];
+  __shared__ int64_t nodes[NEP];
+  const int64_t e = colour_cell(g, colour, blockIdx.x, nw);
+  if (e < 0) return;
+  const int64_t cx = e % g.nc, cy = (e / g.nc) % g.nc, cz = e / ((int64_t)g.nc * g.nc);
+  const int tid = threadIdx.x;
+  for (int l = tid; l < NEP; l += kThreads) {
+    int64_t n = l < NE ? cell_node(g, cx, cy, cz, l) : 0;
+    nodes[l] = n;
+    sc[l] = l < NE ? S[n] : T(0);
+  }
+  __syncthreads();
+  // gather S X into LDS (zero padding rows / columns)
+  for (int i = tid; i < NEP * BT; i += kThreads) {
+    const int l = i / BT, j = i % BT;
+    T v = T(0);
+    if (l < NE && j < B) v = sc[l] * X[nodes[l] * B + j];
+    xs[l * LDB + j] = v;
+  }
+  __syncthreads();
+  const int wave = tid / 64, lane = tid % 64;
+  const int lr = lane % 16, lk = lane / 16;
+  typedef typename Mfma<T>::acc_t acc_t;
+  typedef typename Vec4<T>::type vec_t;
+  const vec_t* Ht = reinterpret_cast<const vec_t*>(H + (size_t)e * NEP * NEP);
+  for (int rt = wave; rt < RT; rt += 4) {
+    acc_t acc[NCT];
+#pragma unroll
+    for (int ct = 0; ct < NCT; ++ct) acc[ct] = acc_t{0, 0, 0, 0};
+    const vec_t* row = Ht + (size_t)rt * RT * 64 + lane;
+    vec_t a = row[0];
+#pragma unroll 1
+    for (int kg = 0; kg < RT; ++kg) {
+      vec_t an = a;
+      if (kg + 1 < RT) an = row[(kg + 1) * 64];
+      const T* xk = xs + (kg * 16 + lk) * LDB + lr;
+      const T av[4] = {a.x, a.y, a.z, a.w};
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+#pragma unroll
+        for (int ct = 0; ct < NCT; ++ct)
+          acc[ct] = Mfma<T>::run(av[s], xk[s * 4 * LDB + ct * 16], acc[ct]);
+      }
+      a = an;
+    }
+    // scatter: lane holds rows rt*16 + Mfma<T>::row(lk, i), column ct*16 + lr
+#pragma unroll
+    for (int ct = 0; ct < NCT; ++ct) {
+      const int col = ct * 16 + lr;
+      if (col >= B) continue;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int r = rt * 16 + Mfma<T>::row(lk, i);
+        if (r < NE) {
+          T* yp = Y + nodes[r] * B + col;
+          *yp += alpha * sc[r] * acc[ct][i];
+        }
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------ Hamiltonian build
+// H_e = K_e/2 + V_e,  V_e = N^T diag(v JxW) N by sum factorisation.
+template <int P>
+__global__ __launch_bounds__(kThreads) void k_build_h(Geo g, const double* __restrict__ Lq,
+                                                      const double* __restrict__ M1,
+                                                      const double* __restrict__ D1, double hh,
+                                                      const double* __restrict__ veff,
+                                                      const double* __restrict__ jxw,
+                                                      double* __restrict__ h64,
+                                                      float* __restrict__ h32) {
+  constexpr int N1 = P + 1, Q = P + 3, NE = ne_of(P), NEP = nep_of(P), RT = NEP / 16;
+  constexpr int N2 = N1 * N1;
+  __shared__ double W[Q * Q * Q];
+  __shared__ double Pq[Q * N2];         // Pq[q][a][a'] = L_a(q) L_a'(q)
+  __shared__ double T1[Q * Q * N2];     // [qz][qy][a][a']
+  __shared__ double T2[Q * N2 * N2];    // [qz][b][b'][a][a']
+  __shared__ double sM[N2], sD[N2];
+  const int64_t e = blockIdx.x;
+  const int tid = threadIdx.x;
+  for (int i = tid; i < Q * Q * Q; i += kThreads) W[i] = veff[e * Q * Q * Q + i] * jxw[i];
+  for (int i = tid; i < Q * N2; i += kThreads) {
+    const int q = i / N2, a = (i / N1) % N1, a2 = i % N1;
+    Pq[i] = Lq[q * N1 + a] * Lq[q * N1 + a2];
+  }
+  for (int i = tid; i < N2; i += kThreads) { sM[i] = M1[i]; sD[i] = D1[i]; }
+  __syncthreads();
+  for (int i = tid; i < Q * Q * N2; i += kThreads) {
+    const int aa = i % N2, qzy = i / N2;
+    double s = 0.0;
+#pragma unroll
+    for (int qx = 0; qx < Q; ++qx) s += Pq[qx * N2 + aa] * W[qzy * Q + qx];
+    T1[i] = s;
+  }
+  __syncthreads();
+  for (int i = tid; i < Q * N2 * N2; i += kThreads) {
+    const int aa = i % N2, bb = (i / N2) % N2, qz = i / (N2 * N2);
+    double s = 0.0;
+#pragma unroll
+    for (int qy = 0; qy < Q; ++qy) s += Pq[qy * N2 + bb] * T1[(qz * Q + qy) * N2 + aa];
+    T2[i] = s;
+  }
+  __syncthreads();
+  double* o64 = h64 + (size_t)e * NEP * NEP;
+  float* o32 = h32 ? h32 + (size_t)e * NEP * NEP : nullptr;
+  for (int idx = tid; idx < NEP * NEP; idx += kThreads) {
+    const int s = idx % 4, lane = (idx / 4) % 64, blk = idx / 256;
+    const int kg = blk % RT, rt = blk / RT;
+    const int row = rt * 16 + lane % 16, col = kg * 16 + s * 4 + lane / 16;
+    double v = 0.0;
+    if (row < NE && col < NE) {
+      const int a = row % N1, b = (row / N1) % N1, c = row / N2;
+      const int a2 = col % N1, b2 = (col / N1) % N1, c2 = col / N2;
+      const int aa = a * N1 + a2, bb = b * N1 + b2, cc = c * N1 + c2;
+      double vv = 0.0;
+#pragma unroll
+      for (int qz = 0; qz < Q; ++qz) vv += Pq[qz * N2 + cc] * T2[(qz * N2 + bb) * N2 + aa];
+      const double k = hh * (sD[aa] * sM[bb] * sM[cc] + sM[aa] * sD[bb] * sM[cc] +
+                             sM[aa] * sM[bb] * sD[cc]);
+      v = 0.5 * k + vv;
+    }
+    o64[idx] = v;
+    if (o32) o32[idx] = (float)v;
+  }
+}
+
+// ------------------------------------------------------ tensor contraction
+// in [N1][N1][N1][NC] -> out [Q][Q][Q][NC] with A [Q][N1] (all in LDS).
+template <int N1, int Q, int NC>
+__device__ void tp_eval(const double* A, const double* in, double* t1, double* t2, double* out) {
+  const int tid = threadIdx.x;
+  for (int i = tid; i < N1 * N1 * Q * NC; i += kThreads) {
+    const int col = i % NC, qx = (i / NC) % Q, cb = i / (NC * Q);
+    const double* src = in + cb * N1 * NC + col;
+    double s = 0.0;
+#pragma unroll
+    for (int a = 0; a < N1; ++a) s += A[qx * N1 + a] * src[a * NC];
+    t1[i] = s;
+  }
+  __syncthreads();
+  for (int i = tid; i < N1 * Q * Q * NC; i += kThreads) {
+    const int col = i % NC, qx = (i / NC) % Q, qy = (i / (NC * Q)) % Q, c = i / (NC * Q * Q);
+    const double* src = t1 + (c * N1 * Q + qx) * NC + col;
+    double s = 0.0;
+#pragma unroll
+    for (int b = 0; b < N1; ++b) s += A[qy * N1 + b] * src[b * Q * NC];
+    t2[i] = s;
+  }
+  __syncthreads();
+  for (int i = tid; i < Q * Q * Q * NC; i += kThreads) {
+    const int col = i % NC, qxy = (i / NC) % (Q * Q), qz = i / (NC * Q * Q);
+    const double* src = t2 + qxy * NC + col;
+    double s = 0.0;
+#pragma unroll
+    for (int c = 0; c < N1; ++c) s += A[qz * N1 + c] * src[c * Q * Q * NC];
+    out[i] = s;
+  }
+  __syncthreads();
+}
+
+// Transpose: in [Q][Q][Q][NC] -> out [N1][N1][N1][NC] (integration against N).
+template <int N1, int Q, int NC>
+__device__ void tp_integrate(const double* A, const double* in, double* t1, double* t2,
+                             double* out) {
+  const int tid = threadIdx.x;
+  for (int i = tid; i < Q * Q * N1 * NC; i += kThreads) {  // t1[qz][qy][a][col]
+    const int col = i % NC, a = (i / NC) % N1, qzy = i / (NC * N1);
+    const double* src = in + qzy * Q * NC + col;
+    double s = 0.0;
+#pragma unroll
+    for (int qx = 0; qx < Q; ++qx) s += A[qx * N1 + a] * src[qx * NC];
+    t1[i] = s;
+  }
+  __syncthreads();
+  for (int i = tid; i < Q * N1 * N1 * NC; i += kThreads) {  // t2[qz][b][a][col]
+    const int col = i % NC, a = (i / NC) % N1, b = (i / (NC * N1)) % N1, qz = i / (NC * N1 * N1);
+    const double* src = t1 + (qz * Q * N1 + a) * NC + col;
+    double s = 0.0;
+#pragma unroll
+    for (int qy = 0; qy < Q; ++qy) s += A[qy * N1 + b] * src[qy * N1 * NC];
+    t2[i] = s;
+  }
+  __syncthreads();
+  for (int i = tid; i < N1 * N1 * N1 * NC; i += kThreads) {  // out[c][b][a][col]
+    const int col = i % NC, ba = (i / NC) % (N1 * N1), c = i / (NC * N1 * N1);
+    const double* src = t2 + ba * NC + col;
+    double s = 0.0;
+#pragma unroll
+    for (int qz = 0; qz < Q; ++qz) s += A[qz * N1 + c] * src[qz * N1 * N1 * NC];
+    out[i] = s;
+  }
+  __syncthreads();
+}
+
+template <int N1, int Q, int NC>
+struct TpScratch {
+  static constexpr int kIn = (N1 * N1 * N1 > Q * Q * Q ? N1 * N1 * N1 : Q * Q * Q) * NC;
+  static constexpr int kT1 = (N1 * N1 * Q > Q * Q * N1 ? N1 * N1 * Q : Q * Q * N1) * NC;
+  static constexpr int kT2 = (N1 * Q * Q > Q * N1 * N1 ? N1 * Q * Q : Q * N1 * N1) * NC;
+  static constexpr int kOut = kIn;
+};
+
+// -------------------------------------------------------------- density
+template <int P, int NC>
+__global__ __launch_bounds__(kThreads) void k_density(Geo g, const double* __restrict__ Lq,
+                                                      const double* __restrict__ X,
+                                                      const double* __restrict__ S, int B,
+                                                      const double* __restrict__ occ,
+                                                      double* __restrict__ rho) {
+  constexpr int N1 = P + 1, Q = P + 3, NE = ne_of(P), NQ = Q * Q * Q;
+  typedef TpScratch<N1, Q, NC> Sz;
+  __shared__ double A[Q * N1];
+  __shared__ double sin_[Sz::kIn], t1[Sz::kT1], t2[Sz::kT2], sout[Sz::kOut];
+  __shared__ double acc[NQ];
+  __shared__ int64_t nodes[NE];
+  const int64_t e = blockIdx.x;
+  const int64_t cx = e % g.nc, cy = (e / g.nc) % g.nc, cz = e / ((int64_t)g.nc * g.nc);
+  const int tid = threadIdx.x;
+  for (int i = tid; i < Q * N1; i += kThreads) A[i] = Lq[i];
+  for (int l = tid; l < NE; l += kThreads) nodes[l] = cell_node(g, cx, cy, cz, l);
+  for (int q = tid; q < NQ; q += kThreads) acc[q] = 0.0;
+  __syncthreads();
+  for (int j0 = 0; j0 < B; j0 += NC) {
+    for (int i = tid; i < NE * NC; i += kThreads) {
+      const int l = i / NC, k = i % NC, j = j0 + k;
+      sin_[i] = j < B ? S[nodes[l]] * X[nodes[l] * B + j] : 0.0;
+    }
+    __syncthreads();
+    tp_eval<N1, Q, NC>(A, sin_, t1, t2, sout);
+    for (int q = tid; q < NQ; q += kThreads) {
+      double s = 0.0;
+#pragma unroll
+      for (int k = 0; k < NC; ++k) {
+        const int j = j0 + k;
+        if (j < B) { const double v = sout[q * NC + k]; s += 2.0 * occ[j] * v * v; }
+      }
+      acc[q] += s;
+    }
+    __syncthreads();
+  }
+  for (int q = tid; q < NQ; q += kThreads) rho[e * NQ + q] = acc[q];
+}
+
+// ---------------------------------------------------- pair quadrature dots
+// out[cell][k] = sum_q wq[cell][q] A_{i_k}(q) B_{j_k}(q)
+template <int P, int NC>
+__global__ __launch_bounds__(kThreads) void k_pair_quad(Geo g, const double* __restrict__ Lq,
+                                                        const double* __restrict__ Av,
+                                                        const double* __restrict__ Bv, int B,
+                                                        const int* __restrict__ pairs, int np,
+                                                        const double* __restrict__ wq,
+                                                        int64_t wstride,
+                                                        double* __restrict__ out) {
+  constexpr int N1 = P + 1, Q = P + 3, NE = ne_of(P), NQ = Q * Q * Q, H = NC / 2;
+  typedef TpScratch<N1, Q, NC> Sz;
+  __shared__ double A[Q * N1];
+  __shared__ double sin_[Sz::kIn], t1[Sz::kT1], t2[Sz::kT2], sout[Sz::kOut];
+  __shared__ double w[NQ];
+  __shared__ int64_t nodes[NE];
+  __shared__ double red[kThreads / 64][H];
+  const int64_t e = blockIdx.x;
+  const int64_t cx = e % g.nc, cy = (e / g.nc) % g.nc, cz = e / ((int64_t)g.nc * g.nc);
+  const int tid = threadIdx.x;
+  for (int i = tid; i < Q * N1; i += kThreads) A[i] = Lq[i];
+  for (int l = tid; l < NE; l += kThreads) nodes[l] = cell_node(g, cx, cy, cz, l);
+  for (int q = tid; q < NQ; q += kThreads) w[q] = wq[e * wstride + q];
+  __syncthreads();
+  for (int k0 = 0; k0 < np; k0 += H) {
+    // columns 0..H-1 <- A_i, H..NC-1 <- B_j of pairs k0..k0+H-1
+    for (int i = tid; i < NE * NC; i += kThreads) {
+      const int l = i / NC, k = i % NC, kp = k0 + (k % H);
+      double v = 0.0;
+      if (kp < np) {
+        if (k < H) v = Av[nodes[l] * B + pairs[2 * kp]];
+        else v = Bv[nodes[l] * B + pairs[2 * kp + 1]];
+      }
+      sin_[i] = v;
+    }
+    __syncthreads();
+    tp_eval<N1, Q, NC>(A, sin_, t1, t2, sout);
+    double part[H];
+#pragma unroll
+    for (int k = 0; k < H; ++k) part[k] = 0.0;
+    for (int q = tid; q < NQ; q += kThreads) {
+#pragma unroll
+      for (int k = 0; k < H; ++k) part[k] += w[q] * sout[q * NC + k] * sout[q * NC + H + k];
+    }
+#pragma unroll
+    for (int k = 0; k < H; ++k) {
+      double v = part[k];
+      for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+      if ((tid & 63) == 0) red[tid / 64][k] = v;
+    }
+    __syncthreads();
+    if (tid < H && k0 + tid < np) {
+      double s = 0.0;
+      for (int wv = 0; wv < kThreads / 64; ++wv) s += red[wv][tid];
+      out[e * np + k0 + tid] = s;
+    }
+    __syncthreads();
+  }
+}
+
+// --------------------------------------------- quadrature cell operator
+// Y += N^T diag(wq) N X on the cells of one colour (no write conflicts).
+template <int P, int NC>
+__global__ __launch_bounds__(kThreads) void k_quad_operator(Geo g, const double* __restrict__ Lq,
+                                                            const double* __restrict__ X, int B,
+                                                            const double* __restrict__ wq,
+                                                            double* __restrict__ Y, int colour,
+                                                            int64_t nw) {
+  constexpr int N1 = P + 1, Q = P + 3, NE = ne_of(P), NQ = Q * Q * Q;
+  typedef TpScratch<N1, Q, NC> Sz;
+  __shared__ double A[Q * N1];
+  __shared__ double sin_[Sz::kIn], t1[Sz::kT1], t2[Sz::kT2], sout[Sz::kOut];
+  __shared__ int64_t nodes[NE];
+  const int64_t e = colour_cell(g, colour, blockIdx.x, nw);
+  if (e < 0) return;
+  const int64_t cx = e % g.nc, cy = (e / g.nc) % g.nc, cz = e / ((int64_t)g.nc * g.nc);
+  const int tid = threadIdx.x;
+  for (int i = tid; i < Q * N1; i += kThreads) A[i] = Lq[i];
+  for (int l = tid; l < NE; l += kThreads) nodes[l] = cell_node(g, cx, cy, cz, l);
+  __syncthreads();
+  for (int j0 = 0; j0 < B; j0 += NC) {
+    for (int i = tid; i < NE * NC; i += kThreads) {
+      const int l = i / NC, j = j0 + i % NC;
+      sin_[i] = j < B ? X[nodes[l] * B + j] : 0.0;
+    }
+    __syncthreads();
+    tp_eval<N1, Q, NC>(A, sin_, t1, t2, sout);
+    for (int i = tid; i < NQ * NC; i += kThreads) sout[i] *= wq[e * NQ + i / NC];
+    __syncthreads();
+    tp_integrate<N1, Q, NC>(A, sout, t1, t2, sin_);
+    for (int i = tid; i < NE * NC; i += kThreads) {
+      const int l = i / NC, j = j0 + i % NC;
+      if (j < B) Y[nodes[l] * B + j] += sin_[i];
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------ v_xc -> quad
+// Child Q1 field (3 child nodes per parent axis) to the parent Gauss points.
+template <int P>
+__global__ __launch_bounds__(kThreads) void k_vxc_to_quad(Geo g, const double* __restrict__ Wq,
+                                                          const double* __restrict__ pot,
+                                                          const double* __restrict__ potBase,
+                                                          double* __restrict__ veff) {
+  constexpr int Q = P + 3, NQ = Q * Q * Q;
+  typedef TpScratch<3, Q, 1> Sz;
+  __shared__ double A[Q * 3];
+  __shared__ double sin_[Sz::kIn], t1[Sz::kT1], t2[Sz::kT2], sout[Sz::kOut];
+  const int64_t e = blockIdx.x;
+  const int64_t cx = e % g.nc, cy = (e / g.nc) % g.nc, cz = e / ((int64_t)g.nc * g.nc);
+  const int tid = threadIdx.x;
+  for (int i = tid; i < Q * 3; i += kThreads) A[i] = Wq[i];
+  for (int i = tid; i < 27; i += kThreads) {
+    const int a = i % 3, b = (i / 3) % 3, c = i / 9;
+    sin_[i] = pot[((2 * cz + c) * g.NV1 + (2 * cy + b)) * (int64_t)g.NV1 + (2 * cx + a)];
+  }
+  __syncthreads();
+  tp_eval<3, Q, 1>(A, sin_, t1, t2, sout);
+  for (int q = tid; q < NQ; q += kThreads) veff[e * NQ + q] = potBase[e * NQ + q] + sout[q];
+}
+
+// ------------------------------------------------------------------ force
+// S(q) = sum_j Psi_j(q) P_j(q) at Q points per axis (child: Q=4 via Ac; parent:
+// Q=p+3 via Lq), then force[child node] += sum_q W(q) N^child(q) S(q).
+template <int P, int Q, int NC>
+__global__ __launch_bounds__(kThreads) void k_force(Geo g, const double* __restrict__ Aeval,
+                                                    const double* __restrict__ Wchild,
+                                                    const double* __restrict__ Psi,
+                                                    const double* __restrict__ Pv, int B,
+                                                    double jxwChild,
+                                                    const double* __restrict__ jxwParent,
+                                                    double* __restrict__ force, int colour,
+                                                    int64_t nw) {
+  constexpr int N1 = P + 1, NE = ne_of(P), NQ = Q * Q * Q, H = NC / 2;
+  typedef TpScratch<N1, Q, NC> Sz;
+  __shared__ double A[Q * N1];
+  __shared__ double Wc[Q * 3];
+  __shared__ double sin_[Sz::kIn], t1[Sz::kT1], t2[Sz::kT2], sout[Sz::kOut];
+  __shared__ double Sq[NQ];
+  __shared__ double u1[3 * 3 * Q], u2[3 * 3 * 3];
+  __shared__ double v1[Q * Q * 3];
+  __shared__ int64_t nodes[NE];
+  const int64_t e = colour_cell(g, colour, blockIdx.x, nw);
+  if (e < 0) return;
+  const int64_t cx = e % g.nc, cy = (e / g.nc) % g.nc, cz = e / ((int64_t)g.nc * g.nc);
+  const int tid = threadIdx.x;
+  for (int i = tid; i < Q * N1; i += kThreads) A[i] = Aeval[i];
+  for (int i = tid; i < Q * 3; i += kThreads) Wc[i] = Wchild[i];
+  for (int l = tid; l < NE; l += kThreads) nodes[l] = cell_node(g, cx, cy, cz, l);
+  for (int q = tid; q < NQ; q += kThreads) Sq[q] = 0.0;
+  __syncthreads();
+  for (int j0 = 0; j0 < B; j0 += H) {
+    for (int i = tid; i < NE * NC; i += kThreads) {
+      const int l = i / NC, k = i % NC, j = j0 + (k % H);
+      double v = 0.0;
+      if (j < B) v = (k < H) ? Psi[nodes[l] * B + j] : Pv[nodes[l] * B + j];
+      sin_[i] = v;
+    }
+    __syncthreads();
+    tp_eval<N1, Q, NC>(A, sin_, t1, t2, sout);
+    for (int q = tid; q < NQ; q += kThreads) {
+      double s = 0.0;
+#pragma unroll
+      for (int k = 0; k < H; ++k) s += sout[q * NC + k] * sout[q * NC + H + k];
+      Sq[q] += s;
+    }
+    __syncthreads();
+  }
+  for (int q = tid; q < NQ; q += kThreads) Sq[q] *= jxwParent ? jxwParent[q] : jxwChild;
+  __syncthreads();
+  // integrate against the child basis: out[c][b][a] (3 child nodes per axis)
+  for (int i = tid; i < Q * Q * 3; i += kThreads) {  // v1[qz][qy][a]
+    const int a = i % 3, qzy = i / 3;
+    double s = 0.0;
+    for (int qx = 0; qx < Q; ++qx) s += Wc[qx * 3 + a] * Sq[qzy * Q + qx];
+    v1[i] = s;
+  }
+  __syncthreads();
+  for (int i = tid; i < Q * 9; i += kThreads) {  // u1[qz][b][a]
+    const int a = i % 3, b = (i / 3) % 3, qz = i / 9;
+    double s = 0.0;
+    for (int qy = 0; qy < Q; ++qy) s += Wc[qy * 3 + b] * v1[(qz * Q + qy) * 3 + a];
+    u1[i] = s;
+  }
+  __syncthreads();
+  for (int i = tid; i < 27; i += kThreads) {
+    const int ba = i % 9, c = i / 9;
+    double s = 0.0;
+    for (int qz = 0; qz < Q; ++qz) s += Wc[qz * 3 + c] * u1[qz * 9 + ba];
+    u2[i] = s;
+  }
+  __syncthreads();
+  for (int i = tid; i < 27; i += kThreads) {
+    const int a = i % 3, b = (i / 3) % 3, c = i / 9;
+    force[((2 * cz + c) * g.NV1 + (2 * cy + b)) * (int64_t)g.NV1 + (2 * cx + a)] += u2[i];
+  }
+}
+
+// --------------------------------------------------------- vector kernels
+// Y = beta Y + ns[n] (a ca[j] X + b cb[j] Z + c cc[j] W); null coefficient
+// pointers read as 1, null operands as 0.
+template <typename T>
+__global__ void k_lincomb(int64_t N, int B, T* __restrict__ Y, double beta,
+                          const T* __restrict__ X, double a, const double* __restrict__ ca,
+                          const T* __restrict__ Z, double b, const double* __restrict__ cb,
+                          const T* __restrict__ W, double c, const double* __restrict__ cc,
+                          const double* __restrict__ ns) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N * B;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int j = (int)(i % B);
+    double v = 0.0;
+    if (X) v = a * (ca ? ca[j] : 1.0) * (double)X[i];
+    if (Z) v += b * (cb ? cb[j] : 1.0) * (double)Z[i];
+    if (W) v += c * (cc ? cc[j] : 1.0) * (double)W[i];
+    if (ns) v *= ns[i / B];
+    if (beta != 0.0) v += beta * (double)Y[i];
+    Y[i] = (T)v;
+  }
+}
+
+__global__ void k_d2f(int64_t n, float* __restrict__ y, const double* __restrict__ x) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = (float)x[i];
+}
+__global__ void k_f2d(int64_t n, double* __restrict__ y, const float* __restrict__ x) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = (double)x[i];
+}
+
+// Column-wise partial dot products.  Thread t of a block owns column t % B and
+// strides over nodes; FP64 accumulation; block results reduced in LDS.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_coldot(int64_t N, int B, const T* __restrict__ X,
+                                                     const T* __restrict__ Y,
+                                                     const double* __restrict__ w,
+                                                     double* __restrict__ partial) {
+  __shared__ double red[kThreads];
+  const int tid = threadIdx.x;
+  const int per = kThreads / B;  // nodes per block step (B <= 256)
+  const int col = tid % B, off = tid / B;
+  double s = 0.0;
+  if (off < per) {
+    for (int64_t n = (int64_t)blockIdx.x * per + off; n < N; n += (int64_t)gridDim.x * per) {
+      const double x = (double)X[n * B + col], y = (double)Y[n * B + col];
+      s += w ? w[n] * x * y : x * y;
+    }
+  }
+  red[tid] = s;
+  __syncthreads();
+  if (tid < B) {
+    double t = 0.0;
+    for (int k = 0; k < per; ++k) t += red[k * B + tid];
+    partial[(int64_t)blockIdx.x * B + tid] = t;
+  }
+}
+
+__global__ void k_sum_partials(int nblk, int B, const double* __restrict__ partial,
+                               double* __restrict__ out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= B) return;
+  double s = 0.0;
+  for (int b = 0; b < nblk; ++b) s += partial[(int64_t)b * B + j];
+  out[j] = s;
+}
+
+// Gram partials: each block accumulates G[i][j] over a strided set of node
+// chunks; rows of X and Y are staged in LDS.
+constexpr int kGramChunk = 32;
+__global__ __launch_bounds__(kThreads) void k_gram(int64_t N, int B, const double* __restrict__ X,
+                                                   const double* __restrict__ Y,
+                                                   double* __restrict__ partial) {
+  extern __shared__ double lds[];
+  double* xs = lds;
+  double* ys = lds + kGramChunk * B;
+  const int tid = threadIdx.x;
+  const int BB = B * B;
+  double acc[32];
+  const int nmine = (BB + kThreads - 1) / kThreads;  // <= 32 for B <= 90
+  for (int k = 0; k < nmine; ++k) acc[k] = 0.0;
+  for (int64_t n0 = (int64_t)blockIdx.x * kGramChunk; n0 < N; n0 += (int64_t)gridDim.x * kGramChunk) {
+    const int cnt = (int)((N - n0) < kGramChunk ? (N - n0) : kGramChunk);
+    __syncthreads();
+    for (int i = tid; i < kGramChunk * B; i += kThreads) {
+      const int r = i / B;
+      xs[i] = r < cnt ? X[n0 * B + i] : 0.0;
+      ys[i] = r < cnt ? Y[n0 * B + i] : 0.0;
+    }
+    __syncthreads();
+    for (int k = 0; k < nmine; ++k) {
+      const int ij = tid + k * kThreads;
+      if (ij >= BB) break;
+      const int i = ij / B, j = ij % B;
+      double s = 0.0;
+      for (int r = 0; r < cnt; ++r) s += xs[r * B + i] * ys[r * B + j];
+      acc[k] += s;
+    }
+  }
+  for (int k = 0; k < nmine; ++k) {
+    const int ij = tid + k * kThreads;
+    if (ij < BB) partial[(int64_t)blockIdx.x * BB + ij] = acc[k];
+  }
+}
+
+// Y = beta Y + ns[n] (X Q)[n]  (X != Y)
+__global__ void k_right_mult(int64_t N, int B, const double* __restrict__ X,
+                             const double* __restrict__ Q, double* __restrict__ Y,
+                             const double* __restrict__ ns, double beta) {
+  extern __shared__ double qs[];
+  for (int i = threadIdx.x; i < B * B; i += blockDim.x) qs[i] = Q[i];
+  __syncthreads();
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < N * B;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t n = t / B;
+    const int j = (int)(t % B);
+    const double* xr = X + n * B;
+    double s = 0.0;
+    for (int i = 0; i < B; ++i) s += xr[i] * qs[i * B + j];
+    if (ns) s *= ns[n];
+    Y[t] = beta != 0.0 ? beta * Y[t] + s : s;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_residual(int64_t N, int B, const T* __restrict__ HX,
+                                                       const T* __restrict__ X,
+                                                       const double* __restrict__ ev,
+                                                       double* __restrict__ partial) {
+  __shared__ double red[kThreads];
+  const int tid = threadIdx.x;
+  const int per = kThreads / B;
+  const int col = tid % B, off = tid / B;
+  double s = 0.0;
+  if (off < per) {
+    const double lam = ev[col];
+    for (int64_t n = (int64_t)blockIdx.x * per + off; n < N; n += (int64_t)gridDim.x * per) {
+      const double r = (double)HX[n * B + col] - lam * (double)X[n * B + col];
+      s += r * r;
+    }
+  }
+  red[tid] = s;
+  __syncthreads();
+  if (tid < B) {
+    double t = 0.0;
+    for (int k = 0; k < per; ++k) t += red[k * B + tid];
+    partial[(int64_t)blockIdx.x * B + tid] = t;
+  }
+}
+
+__global__ void k_loss(int64_t nQ, int nq, const double* __restrict__ rho,
+                       const double* __restrict__ rhoT, const double* __restrict__ w,
+                       const double* __restrict__ veff, const double* __restrict__ jxw,
+                       double* __restrict__ u, double* __restrict__ partial) {
+  __shared__ double red[4][kThreads];
+  const int tid = threadIdx.x;
+  double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+  for (int64_t q = blockIdx.x * (int64_t)kThreads + tid; q < nQ; q += (int64_t)gridDim.x * kThreads) {
+    const double jw = jxw[q % nq];
+    const double rd = rhoT[q] - 0.5 * rho[q];
+    u[q] = rd * w[q];
+    s0 += rd * rd * w[q] * jw;
+    s1 += rd * rd * jw;
+    s2 += fabs(rd * jw);
+    s3 += veff[q] * rho[q] * jw;
+  }
+  red[0][tid] = s0; red[1][tid] = s1; red[2][tid] = s2; red[3][tid] = s3;
+  __syncthreads();
+  for (int st = kThreads / 2; st > 0; st >>= 1) {
+    if (tid < st)
+      for (int k = 0; k < 4; ++k) red[k][tid] += red[k][tid + st];
+    __syncthreads();
+  }
+  if (tid < 4) partial[blockIdx.x * 4 + tid] = red[tid][0];
+}
+
+__global__ void k_zero_rows(int64_t N, int B, double* __restrict__ X,
+                            const unsigned char* __restrict__ mask) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < N * B;
+       t += (int64_t)gridDim.x * blockDim.x)
+    if (mask[t / B]) X[t] = 0.0;
+}
+
+
+// ------------------------------------------------------------- MINRES
+// Per-column state st[k * B + j] (see MinresSlot); one thread per column.
+__global__ void k_minres_start(int B, int nblk, const double* __restrict__ partial,
+                               double* __restrict__ st) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= B) return;
+  double s = 0.0;
+  for (int b = 0; b < nblk; ++b) s += partial[(int64_t)b * B + j];
+  const double beta = sqrt(fmax(s, 0.0));
+  st[MS_BETA * B + j] = beta;
+  st[MS_OLDB * B + j] = 0.0;
+  st[MS_DBAR * B + j] = 0.0;
+  st[MS_EPSLN * B + j] = 0.0;
+  st[MS_PHIBAR * B + j] = beta;
+  st[MS_CS * B + j] = -1.0;
+  st[MS_SN * B + j] = 0.0;
+  st[MS_IT * B + j] = 0.0;
+  st[MS_BETA1 * B + j] = beta;
+  const bool done = !(beta > st[MS_TOL * B + j]);
+  st[MS_DONE * B + j] = done ? 1.0 : 0.0;
+  st[MS_CV * B + j] = done ? 0.0 : 1.0 / beta;
+  st[MS_CR1 * B + j] = 0.0;
+}
+
+__global__ void k_minres_alpha(int B, int nblk, const double* __restrict__ partial,
+                               double* __restrict__ st) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= B) return;
+  double s = 0.0;
+  for (int b = 0; b < nblk; ++b) s += partial[(int64_t)b * B + j];
+  st[MS_ALFA * B + j] = s;
+  const bool done = st[MS_DONE * B + j] != 0.0;
+  st[MS_CA2 * B + j] = done ? 0.0 : -s / st[MS_BETA * B + j];
+}
+
+// r_new = y + ca2 r2 (stored in rnew), y <- D r_new, partial dot(r_new, D r_new)
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_minres_lanczos(int64_t N, int B, T* __restrict__ y,
+                                                             const T* __restrict__ r2,
+                                                             T* __restrict__ rnew,
+                                                             const double* __restrict__ D,
+                                                             const double* __restrict__ st,
+                                                             double* __restrict__ partial) {
+  __shared__ double red[kThreads];
+  const int tid = threadIdx.x;
+  const int per = kThreads / B;
+  const int col = tid % B, off = tid / B;
+  double s = 0.0;
+  if (off < per) {
+    const double c = st[MS_CA2 * B + col];
+    for (int64_t n = (int64_t)blockIdx.x * per + off; n < N; n += (int64_t)gridDim.x * per) {
+      const int64_t i = n * B + col;
+      const double rn = (double)y[i] + c * (double)r2[i];
+      const T rt = (T)rn;
+      const double z = D[n] * (double)rt;
+      rnew[i] = rt;
+      y[i] = (T)z;
+      s += (double)rt * z;
+    }
+  }
+  red[tid] = s;
+  __syncthreads();
+  if (tid < B) {
+    double t = 0.0;
+    for (int k = 0; k < per; ++k) t += red[k * B + tid];
+    partial[(int64_t)blockIdx.x * B + tid] = t;
+  }
+}
+
+__global__ void k_minres_givens(int B, int nblk, const double* __restrict__ partial,
+                                double* __restrict__ st) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= B) return;
+  double* S = st;
+  if (S[MS_DONE * B + j] != 0.0) {
+    S[MS_CWV * B + j] = 0.0; S[MS_CWW1 * B + j] = 0.0; S[MS_CWW * B + j] = 0.0;
+    S[MS_CX * B + j] = 0.0; S[MS_CV * B + j] = 0.0; S[MS_CR1 * B + j] = 0.0;
+    return;
+  }
+  double s = 0.0;
+  for (int b = 0; b < nblk; ++b) s += partial[(int64_t)b * B + j];
+  const double oldb = S[MS_BETA * B + j];
+  const double beta = sqrt(fmax(s, 0.0));
+  const double alfa = S[MS_ALFA * B + j];
+  const double cs = S[MS_CS * B + j], sn = S[MS_SN * B + j], dbar = S[MS_DBAR * B + j];
+  const double oldeps = S[MS_EPSLN * B + j];
+  const double delta = cs * dbar + sn * alfa;
+  const double gbar = sn * dbar - cs * alfa;
+  const double epsln = sn * beta;
+  const double ndbar = -cs * beta;
+  const double gamma = fmax(sqrt(gbar * gbar + beta * beta), 1e-300);
+  const double ncs = gbar / gamma, nsn = beta / gamma;
+  const double phibar0 = S[MS_PHIBAR * B + j];
+  const double phi = ncs * phibar0, phibar = nsn * phibar0;
+  S[MS_OLDB * B + j] = oldb;
+  S[MS_BETA * B + j] = beta;
+  S[MS_EPSLN * B + j] = epsln;
+  S[MS_DBAR * B + j] = ndbar;
+  S[MS_CS * B + j] = ncs;
+  S[MS_SN * B + j] = nsn;
+  S[MS_PHIBAR * B + j] = phibar;
+  S[MS_CWV * B + j] = 1.0 / gamma;
+  S[MS_CWW1 * B + j] = -oldeps / gamma;
+  S[MS_CWW * B + j] = -delta / gamma;
+  S[MS_CX * B + j] = phi;
+  S[MS_IT * B + j] += 1.0;
+  const bool done = !(phibar > S[MS_TOL * B + j]) || beta == 0.0;
+  S[MS_DONE * B + j] = done ? 1.0 : 0.0;
+  S[MS_CV * B + j] = done ? 0.0 : 1.0 / beta;
+  S[MS_CR1 * B + j] = done ? 0.0 : -beta / oldb;
+}
+
+// w_new = cwv v + cww1 w1 + cww w  (written over w1);  x += cx w_new
+template <typename T>
+__global__ void k_minres_update(int64_t N, int B, const T* __restrict__ v, T* __restrict__ w1,
+                                const T* __restrict__ w, double* __restrict__ x,
+                                const double* __restrict__ st) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N * B;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int j = (int)(i % B);
+    const double wn = st[MS_CWV * B + j] * (double)v[i] + st[MS_CWW1 * B + j] * (double)w1[i] +
+                      st[MS_CWW * B + j] * (double)w[i];
+    w1[i] = (T)wn;
+    x[i] += st[MS_CX * B + j] * wn;
+  }
+}
+
+__global__ void k_copy_done(int B, const double* __restrict__ st, int* __restrict__ flags) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < B) flags[j] = st[MS_DONE * B + j] != 0.0 ? 1 : 0;
+}
+
+
+// dst[n][j] = src[n][off + j]
+__global__ void k_copy_cols(int64_t N, int Bs, int off, int Bd, const double* __restrict__ src,
+                            double* __restrict__ dst) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N * Bd;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t n = i / Bd;
+    dst[i] = src[n * Bs + off + (int)(i % Bd)];
+  }
+}
+
+__global__ void k_mul_q(int64_t nQ, int nq, const double* __restrict__ u,
+                        const double* __restrict__ jxw, double* __restrict__ out) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nQ;
+       i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = u[i] * jxw[i % nq];
+}
+
+// partial[blk][k] = sum_n w[n] X[n][i_k] Y[n][j_k]   (np <= 256)
+__global__ __launch_bounds__(kThreads) void k_pairdot(int64_t N, int B, const double* __restrict__ X,
+                                                      const double* __restrict__ Y,
+                                                      const double* __restrict__ w,
+                                                      const int* __restrict__ pairs, int np,
+                                                      double* __restrict__ partial) {
+  __shared__ double red[kThreads];
+  const int tid = threadIdx.x;
+  const int per = kThreads / np;
+  const int k = tid % np, off = tid / np;
+  double s = 0.0;
+  if (off < per) {
+    const int i = pairs[2 * k], j = pairs[2 * k + 1];
+    for (int64_t n = (int64_t)blockIdx.x * per + off; n < N; n += (int64_t)gridDim.x * per)
+      s += (w ? w[n] : 1.0) * X[n * B + i] * Y[n * B + j];
+  }
+  red[tid] = s;
+  __syncthreads();
+  if (tid < np) {
+    double t = 0.0;
+    for (int m = 0; m < per; ++m) t += red[m * np + tid];
+    partial[(int64_t)blockIdx.x * np + tid] = t;
+  }
+}
+
+inline int grid_for(int64_t n) {
+  int64_t g = (n + kThreads - 1) / kThreads;
+  if (g > 8192) g = 8192;
+  if (g < 1) g = 1;
+  return (int)g;
+}
+
+}  // namespace
+
+// ============================================================== launchers
+int64_t hamiltonian_tile_elems(int p) { return (int64_t)nep_of(p) * nep_of(p); }
+
+void launch_build_hamiltonian(const Geo& g, const DevTables& t, double hh, const double* veff,
+                              const double* jxw, double* h64, float* h32, hipStream_t s) {
+  dim3 grid((unsigned)g.ncell), block(kThreads);
+  switch (g.p) {
+    case 2: k_build_h<2><<<grid, block, 0, s>>>(g, t.Lq.p, t.M1.p, t.D1.p, hh, veff, jxw, h64, h32); break;
+    case 3: k_build_h<3><<<grid, block, 0, s>>>(g, t.Lq.p, t.M1.p, t.D1.p, hh, veff, jxw, h64, h32); break;
+    case 4: k_build_h<4><<<grid, block, 0, s>>>(g, t.Lq.p, t.M1.p, t.D1.p, hh, veff, jxw, h64, h32); break;
+    case 5: k_build_h<5><<<grid, block, 0, s>>>(g, t.Lq.p, t.M1.p, t.D1.p, hh, veff, jxw, h64, h32); break;
+    default: throw Error(IDFT_EINVAL, "degree not supported by the HIP kernels (2..5)");
+  }
+  IDFT_HIP(hipGetLastError());
+}
+
+template <typename T, int P>
+static void cell_apply_p(const Geo& g, const T* H, const T* S, const T* X, T* Y, int B,
+                         double alpha, hipStream_t s) {
+  const int nct = (B + 15) / 16;
+  for (int colour = 0; colour < 8; ++colour) {
+    const int64_t nw = colour_count(g, colour);
+    if (nw == 0) continue;
+    dim3 grid(colour_grid(nw)), block(kThreads);
+    switch (nct) {
+      case 1: k_cell_apply<T, P, 1><<<grid, block, 0, s>>>(g, H, S, X, Y, B, (T)alpha, colour, nw); break;
+      case 2: k_cell_apply<T, P, 2><<<grid, block, 0, s>>>(g, H, S, X, Y, B, (T)alpha, colour, nw); break;
+      case 3: k_cell_apply<T, P, 3><<<grid, block, 0, s>>>(g, H, S, X, Y, B, (T)alpha, colour, nw); break;
+      case 4: k_cell_apply<T, P, 4><<<grid, block, 0, s>>>(g, H, S, X, Y, B, (T)alpha, colour, nw); break;
+      case 5: k_cell_apply<T, P, 5><<<grid, block, 0, s>>>(g, H, S, X, Y, B, (T)alpha, colour, nw); break;
+      default: throw Error(IDFT_EINVAL, "block wider than 80 columns");
+    }
+  }
+  IDFT_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_cell_apply(const Geo& g, const T* H, const T* S, const T* X, T* Y, int B,
+                       double alpha, hipStream_t s) {
+  switch (g.p) {
+    case 2: cell_apply_p<T, 2>(g, H, S, X, Y, B, alpha, s); break;
+    case 3: cell_apply_p<T, 3>(g, H, S, X, Y, B, alpha, s); break;
+    case 4: cell_apply_p<T, 4>(g, H, S, X, Y, B, alpha, s); break;
+    case 5: cell_apply_p<T, 5>(g, H, S, X, Y, B, alpha, s); break;
+    default: throw Error(IDFT_EINVAL, "degree not supported by the HIP kernels (2..5)");
+  }
+}
+template void launch_cell_apply<float>(const Geo&, const float*, const float*, const float*,
+                                       float*, int, double, hipStream_t);
+template void launch_cell_apply<double>(const Geo&, const double*, const double*, const double*,
+                                        double*, int, double, hipStream_t);
+
+template <typename T>
+void launch_lincomb(int64_t N, int B, T* Y, double beta, const T* X, double a, const double* ca,
+                    const T* Z, double b, const double* cb, const double* ns, hipStream_t s) {
+  k_lincomb<T><<<grid_for(N * B), kThreads, 0, s>>>(N, B, Y, beta, X, a, ca, Z, b, cb, nullptr,
+                                                    0.0, nullptr, ns);
+  IDFT_HIP(hipGetLastError());
+}
+template <typename T>
+void launch_lincomb3(int64_t N, int B, T* Y, double beta, const T* X, double a, const double* ca,
+                     const T* Z, double b, const double* cb, const T* W, double c,
+                     const double* cc, const double* ns, hipStream_t s) {
+  k_lincomb<T><<<grid_for(N * B), kThreads, 0, s>>>(N, B, Y, beta, X, a, ca, Z, b, cb, W, c, cc, ns);
+  IDFT_HIP(hipGetLastError());
+}
+template void launch_lincomb<float>(int64_t, int, float*, double, const float*, double,
+                                    const double*, const float*, double, const double*,
+                                    const double*, hipStream_t);
+template void launch_lincomb<double>(int64_t, int, double*, double, const double*, double,
+                                     const double*, const double*, double, const double*,
+                                     const double*, hipStream_t);
+template void launch_lincomb3<float>(int64_t, int, float*, double, const float*, double,
+                                     const double*, const float*, double, const double*,
+                                     const float*, double, const double*, const double*,
+                                     hipStream_t);
+template void launch_lincomb3<double>(int64_t, int, double*, double, const double*, double,
+                                      const double*, const double*, double, const double*,
+                                      const double*, double, const double*, const double*,
+                                      hipStream_t);
+
+void launch_convert_d2f(int64_t n, float* y, const double* x, hipStream_t s) {
+  k_d2f<<<grid_for(n), kThreads, 0, s>>>(n, y, x);
+  IDFT_HIP(hipGetLastError());
+}
+void launch_convert_f2d(int64_t n, double* y, const float* x, hipStream_t s) {
+  k_f2d<<<grid_for(n), kThreads, 0, s>>>(n, y, x);
+  IDFT_HIP(hipGetLastError());
+}
+
+int coldot_blocks(int64_t N, int B) {
+  const int per = kThreads / B;
+  int64_t g = (N + per - 1) / per;
+  if (g > 1024) g = 1024;
+  return (int)(g < 1 ? 1 : g);
+}
+
+template <typename T>
+int launch_coldot_partial(int64_t N, int B, const T* X, const T* Y, const double* w,
+                          double* partial, hipStream_t s) {
+  if (B > kThreads) throw Error(IDFT_EINVAL, "block too wide for column reductions");
+  const int nb = coldot_blocks(N, B);
+  k_coldot<T><<<nb, kThreads, 0, s>>>(N, B, X, Y, w, partial);
+  IDFT_HIP(hipGetLastError());
+  return nb;
+}
+template int launch_coldot_partial<float>(int64_t, int, const float*, const float*, const double*,
+                                          double*, hipStream_t);
+template int launch_coldot_partial<double>(int64_t, int, const double*, const double*,
+                                           const double*, double*, hipStream_t);
+
+void launch_sum_partials(int nblk, int B, const double* partial, double* out, hipStream_t s) {
+  k_sum_partials<<<(B + 63) / 64, 64, 0, s>>>(nblk, B, partial, out);
+  IDFT_HIP(hipGetLastError());
+}
+
+int gram_partial_blocks(int64_t N, int B) {
+  (void)B;
+  int64_t g = (N + kGramChunk - 1) / kGramChunk;
+  if (g > 1024) g = 1024;
+  return (int)(g < 1 ? 1 : g);
+}
+
+void launch_gram(int64_t N, int B, const double* X, const double* Y, double* partial, double* G,
+                 hipStream_t s) {
+  if (B * B > 32 * kThreads) throw Error(IDFT_EINVAL, "block too wide for Gram kernel");
+  const int nb = gram_partial_blocks(N, B);
+  k_gram<<<nb, kThreads, 2 * kGramChunk * B * sizeof(double), s>>>(N, B, X, Y, partial);
+  IDFT_HIP(hipGetLastError());
+  launch_sum_partials(nb, B * B, partial, G, s);
+}
+
+void launch_right_mult(int64_t N, int B, const double* X, const double* Q, double* Y,
+                       const double* ns, double beta, hipStream_t s) {
+  k_right_mult<<<grid_for(N * B), kThreads, B * B * sizeof(double), s>>>(N, B, X, Q, Y, ns, beta);
+  IDFT_HIP(hipGetLastError());
+}
+
+int launch_residual_partial(int64_t N, int B, const double* HX, const double* X,
+                            const double* ev, double* partial, hipStream_t s) {
+  const int nb = coldot_blocks(N, B);
+  k_residual<double><<<nb, kThreads, 0, s>>>(N, B, HX, X, ev, partial);
+  IDFT_HIP(hipGetLastError());
+  return nb;
+}
+
+void launch_vxc_to_quad(const Geo& g, const DevTables& t, const double* pot,
+                        const double* potBase, double* veff, hipStream_t s) {
+  dim3 grid((unsigned)g.ncell), block(kThreads);
+  switch (g.p) {
+    case 2: k_vxc_to_quad<2><<<grid, block, 0, s>>>(g, t.Wq.p, pot, potBase, veff); break;
+    case 3: k_vxc_to_quad<3><<<grid, block, 0, s>>>(g, t.Wq.p, pot, potBase, veff); break;
+    case 4: k_vxc_to_quad<4><<<grid, block, 0, s>>>(g, t.Wq.p, pot, potBase, veff); break;
+    case 5: k_vxc_to_quad<5><<<grid, block, 0, s>>>(g, t.Wq.p, pot, potBase, veff); break;
+    default: throw Error(IDFT_EINVAL, "degree not supported");
+  }
+  IDFT_HIP(hipGetLastError());
+}
+
+void launch_density(const Geo& g, const DevTables& t, const double* X, const double* S, int B,
+                    const double* occ, double* rho, hipStream_t s) {
+  dim3 grid((unsigned)g.ncell), block(kThreads);
+  switch (g.p) {
+    case 2: k_density<2, 8><<<grid, block, 0, s>>>(g, t.Lq.p, X, S, B, occ, rho); break;
+    case 3: k_density<3, 8><<<grid, block, 0, s>>>(g, t.Lq.p, X, S, B, occ, rho); break;
+    case 4: k_density<4, 8><<<grid, block, 0, s>>>(g, t.Lq.p, X, S, B, occ, rho); break;
+    case 5: k_density<5, 8><<<grid, block, 0, s>>>(g, t.Lq.p, X, S, B, occ, rho); break;
+    default: throw Error(IDFT_EINVAL, "degree not supported");
+  }
+  IDFT_HIP(hipGetLastError());
+}
+
+void launch_loss(int64_t nQ, int nq, const double* rho, const double* rhoT, const double* w,
+                 const double* veff, const double* jxw, double* u, double* partial,
+                 hipStream_t s) {
+  k_loss<<<256, kThreads, 0, s>>>(nQ, nq, rho, rhoT, w, veff, jxw, u, partial);
+  IDFT_HIP(hipGetLastError());
+}
+
+void launch_pair_quad(const Geo& g, const DevTables& t, const double* A, const double* Bv, int B,
+                      const int* pairs, int np, const double* wq, int64_t wstride, double* out,
+                      hipStream_t s) {
+  if (np == 0) return;
+  dim3 grid((unsigned)g.ncell), block(kThreads);
+  switch (g.p) {
+    case 2: k_pair_quad<2, 8><<<grid, block, 0, s>>>(g, t.Lq.p, A, Bv, B, pairs, np, wq, wstride, out); break;
+    case 3: k_pair_quad<3, 8><<<grid, block, 0, s>>>(g, t.Lq.p, A, Bv, B, pairs, np, wq, wstride, out); break;
+    case 4: k_pair_quad<4, 8><<<grid, block, 0, s>>>(g, t.Lq.p, A, Bv, B, pairs, np, wq, wstride, out); break;
+    case 5: k_pair_quad<5, 8><<<grid, block, 0, s>>>(g, t.Lq.p, A, Bv, B, pairs, np, wq, wstride, out); break;
+    default: throw Error(IDFT_EINVAL, "degree not supported");
+  }
+  IDFT_HIP(hipGetLastError());
+}
+
+void launch_quad_operator(const Geo& g, const DevTables& t, const double* X, int B,
+                          const double* wq, double* Y, int colour, hipStream_t s) {
+  const int64_t nw = colour_count(g, colour);
+  if (nw == 0) return;
+  dim3 grid(colour_grid(nw)), block(kThreads);
+  switch (g.p) {
+    case 2: k_quad_operator<2, 8><<<grid, block, 0, s>>>(g, t.Lq.p, X, B, wq, Y, colour, nw); break;
+    case 3: k_quad_operator<3, 8><<<grid, block, 0, s>>>(g, t.Lq.p, X, B, wq, Y, colour, nw); break;
+    case 4: k_quad_operator<4, 8><<<grid, block, 0, s>>>(g, t.Lq.p, X, B, wq, Y, colour, nw); break;
+    case 5: k_quad_operator<5, 8><<<grid, block, 0, s>>>(g, t.Lq.p, X, B, wq, Y, colour, nw); break;
+    default: throw Error(IDFT_EINVAL, "degree not supported");
+  }
+  IDFT_HIP(hipGetLastError());
+}
+
+template <int P>
+static void force_p(const Geo& g, const DevTables& t, const double* Psi, const double* Pv, int B,
+                    int mode, double jxwChild, const double* jxwParent, double* force,
+                    int colour, int64_t nw, hipStream_t s) {
+  dim3 grid(colour_grid(nw)), block(kThreads);
+  if (mode == 0)
+    k_force<P, 4, 8><<<grid, block, 0, s>>>(g, t.Lc.p, t.Wc.p, Psi, Pv, B, jxwChild, nullptr,
+                                            force, colour, nw);
+  else
+    k_force<P, P + 3, 8><<<grid, block, 0, s>>>(g, t.Lq.p, t.Wq.p, Psi, Pv, B, 0.0, jxwParent,
+                                                force, colour, nw);
+}
+
+void launch_force(const Geo& g, const DevTables& t, const double* Psi, const double* P, int B,
+                  int mode, double jxwChild, const double* jxwParent, double* force, int colour,
+                  hipStream_t s) {
+  const int64_t nw = colour_count(g, colour);
+  if (nw == 0) return;
+  switch (g.p) {
+    case 2: force_p<2>(g, t, Psi, P, B, mode, jxwChild, jxwParent, force, colour, nw, s); break;
+    case 3: force_p<3>(g, t, Psi, P, B, mode, jxwChild, jxwParent, force, colour, nw, s); break;
+    case 4: force_p<4>(g, t, Psi, P, B, mode, jxwChild, jxwParent, force, colour, nw, s); break;
+    case 5: force_p<5>(g, t, Psi, P, B, mode, jxwChild, jxwParent, force, colour, nw, s); break;
+    default: throw Error(IDFT_EINVAL, "degree not supported");
+  }
+  IDFT_HIP(hipGetLastError());
+}
+
+void launch_zero_rows(int64_t N, int B, double* X, const unsigned char* mask, hipStream_t s) {
+  k_zero_rows<<<grid_for(N * B), kThreads, 0, s>>>(N, B, X, mask);
+  IDFT_HIP(hipGetLastError());
+}
+
+
+void launch_minres_start(int B, int nblk, const double* partial, double* st, hipStream_t s) {
+  k_minres_start<<<(B + 63) / 64, 64, 0, s>>>(B, nblk, partial, st);
+  IDFT_HIP(hipGetLastError());
+}
+void launch_minres_alpha(int B, int nblk, const double* partial, double* st, hipStream_t s) {
+  k_minres_alpha<<<(B + 63) / 64, 64, 0, s>>>(B, nblk, partial, st);
+  IDFT_HIP(hipGetLastError());
+}
+template <typename T>
+int launch_minres_lanczos(int64_t N, int B, T* y, const T* r2, T* rnew, const double* D,
+                          const double* st, double* partial, hipStream_t s) {
+  const int nb = coldot_blocks(N, B);
+  k_minres_lanczos<T><<<nb, kThreads, 0, s>>>(N, B, y, r2, rnew, D, st, partial);
+  IDFT_HIP(hipGetLastError());
+  return nb;
+}
+template int launch_minres_lanczos<float>(int64_t, int, float*, const float*, float*,
+                                          const double*, const double*, double*, hipStream_t);
+template int launch_minres_lanczos<double>(int64_t, int, double*, const double*, double*,
+                                           const double*, const double*, double*, hipStream_t);
+void launch_minres_givens(int B, int nblk, const double* partial, double* st, hipStream_t s) {
+  k_minres_givens<<<(B + 63) / 64, 64, 0, s>>>(B, nblk, partial, st);
+  IDFT_HIP(hipGetLastError());
+}
+template <typename T>
+void launch_minres_update(int64_t N, int B, const T* v, T* w1, const T* w, double* x,
+                          const double* st, hipStream_t s) {
+  k_minres_update<T><<<grid_for(N * B), kThreads, 0, s>>>(N, B, v, w1, w, x, st);
+  IDFT_HIP(hipGetLastError());
+}
+template void launch_minres_update<float>(int64_t, int, const float*, float*, const float*,
+                                          double*, const double*, hipStream_t);
+template void launch_minres_update<double>(int64_t, int, const double*, double*, const double*,
+                                           double*, const double*, hipStream_t);
+void launch_copy_done(int B, const double* st, int* flags, hipStream_t s) {
+  k_copy_done<<<(B + 63) / 64, 64, 0, s>>>(B, st, flags);
+  IDFT_HIP(hipGetLastError());
+}
+
+
+void launch_copy_cols(int64_t N, int Bs, int off, int Bd, const double* src, double* dst,
+                      hipStream_t s) {
+  k_copy_cols<<<grid_for(N * Bd), kThreads, 0, s>>>(N, Bs, off, Bd, src, dst);
+  IDFT_HIP(hipGetLastError());
+}
+void launch_mul_q(int64_t nQ, int nq, const double* u, const double* jxw, double* out,
+                  hipStream_t s) {
+  k_mul_q<<<grid_for(nQ), kThreads, 0, s>>>(nQ, nq, u, jxw, out);
+  IDFT_HIP(hipGetLastError());
+}
+int launch_pairdot_partial(int64_t N, int B, const double* X, const double* Y, const double* w,
+                           const int* pairs, int np, double* partial, hipStream_t s) {
+  if (np > kThreads) throw Error(IDFT_EINVAL, "too many degenerate pairs");
+  const int nb = coldot_blocks(N, np);
+  k_pairdot<<<nb, kThreads, 0, s>>>(N, B, X, Y, w, pairs, np, partial);
+  IDFT_HIP(hipGetLastError());
+  return nb;
+}
+
+}  // namespace idft

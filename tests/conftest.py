@@ -1,0 +1,24 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU and the HIP library")
+
+
+@pytest.fixture(scope="session")
+def olib():
+    from oracle import oracle
+    return oracle.load()
+
+
+@pytest.fixture(scope="session")
+def glib():
+    import invdft_amd
+    return invdft_amd.load()

@@ -1,0 +1,164 @@
+"""HIP library vs the FP64 CPU oracle on identical inputs (gfx950 only)."""
+import numpy as np
+import pytest
+
+from invdft_amd import abi
+from tests.helpers import bare_pair, rel, small_problem
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print("CASE", *a, flush=True)
+
+
+def test_backend(glib):
+    assert glib.idft_backend() == b"hip-gfx950"
+
+
+@pytest.mark.parametrize("p", [2, 3, 4, 5])
+@pytest.mark.parametrize("ncols", [1, 12, 20, 80])
+def test_cell_apply_fp64(glib, olib, p, ncols):
+    _say("apply_fp64", p, ncols)
+    g, o, s = bare_pair(glib, olib, 3, p, 4)
+    x = np.random.default_rng(p * 100 + ncols).standard_normal((s.n_nodes, ncols))
+    yg = g.apply_hamiltonian(x)
+    yo = o.apply_hamiltonian(x)
+    assert rel(yg, yo) < 1e-13
+
+
+@pytest.mark.parametrize("p", [3, 4, 5])
+@pytest.mark.parametrize("ncols", [1, 16, 40])
+def test_cell_apply_fp32(glib, olib, p, ncols):
+    _say("apply_fp32", p, ncols)
+    g, o, s = bare_pair(glib, olib, 3, p, 4, precision=abi.MIXED)
+    x = np.random.default_rng(p + ncols).standard_normal((s.n_nodes, ncols))
+    yg = g.apply_hamiltonian(x, precision=abi.MIXED)
+    yo = o.apply_hamiltonian(x)
+    assert rel(yg, yo) < 2e-6
+
+
+@pytest.mark.parametrize("prec,tol", [(abi.FP64, 1e-11), (abi.MIXED, 1e-4)])
+def test_chebyshev_filter(glib, olib, prec, tol):
+    _say("filter", prec)
+    g, o, s = bare_pair(glib, olib, 3, 4, 6, precision=prec)
+    x = np.random.default_rng(5).standard_normal((s.n_nodes, 6))
+    a, b, a0 = 2.0, 60.0, -3.0
+    xg = g.chebyshev_filter(x, 12, a, b, a0, precision=prec)
+    xo = o.chebyshev_filter(x, 12, a, b, a0)
+    assert rel(xg, xo) < tol
+
+
+def _subspace_gap(A, B):
+    qa, _ = np.linalg.qr(A)
+    qb, _ = np.linalg.qr(B)
+    return np.linalg.norm(qa @ (qa.T @ qb) - qb)
+
+
+def test_eigensolve_density(glib, olib):
+    _say("eigensolve")
+    prob = small_problem(olib)
+    g = prob.context(glib)
+    o = prob.context(olib)
+    for c in (g, o):
+        c.set_potential(prob.vxc0)
+    eg, rg, _ = g.eigensolve(1e-11, 60)
+    eo, ro, _ = o.eigensolve(1e-11, 60)
+    assert rg < 1e-10 and ro < 1e-10
+    assert np.max(np.abs(eg - eo)) < 1e-10
+    occg, _ = g.occupancies()
+    occo, _ = o.occupancies()
+    assert np.max(np.abs(occg - occo)) < 1e-9
+    assert rel(g.density(), o.density()) < 1e-9
+    nocc = int(np.sum(occo > 0.5))
+    assert _subspace_gap(g.eigenvectors()[:, :nocc], o.eigenvectors()[:, :nocc]) < 1e-8
+
+
+def _adjoint_setup(ctx, prob, psi, occ, eig, u, pairs, fermi):
+    B = psi.shape[1]
+    mpsi = ctx.mv(B, psi)
+    neg = ctx.adj_update_input_psi(mpsi, occ, u, pairs, fermi, eig)
+    return mpsi, neg
+
+
+def test_adjoint_contract(glib, olib):
+    _say("adjoint_contract")
+    prob = small_problem(olib)
+    o = prob.context(olib)
+    o.set_potential(prob.vxc0)
+    ev, _, _ = o.eigensolve(1e-11, 60)
+    occ, fermi = o.occupancies()
+    psi = o.eigenvectors()
+    rho = o.density()
+    u = prob.rho_target_q - 0.5 * rho
+    B = psi.shape[1]
+    pairs = []
+    for i in range(B):
+        for j in range(B):
+            if abs(ev[i] - ev[j]) < 0.002:
+                pairs += [i, j]
+    g = prob.context(glib)
+    g.set_potential(prob.vxc0)
+    res = {}
+    for name, c in (("g", g), ("o", o)):
+        mpsi, neg = _adjoint_setup(c, prob, psi, occ, ev, u, pairs, fermi)
+        assert np.allclose(neg, -ev)
+        x = c.mv(B)
+        rhs = c.mv(B)
+        c.adj_compute_rhs(None, x, rhs)
+        xin = c.mv(B, np.random.default_rng(1).standard_normal((psi.shape[0], B)))
+        ax = c.mv(B)
+        c.adj_vmult(ax, xin)
+        pj = c.mv(B)
+        c.adj_precondition_jacobi(pj, xin)
+        dq = c.adj_dot_quad_wise(mpsi, mpsi, len(pairs) // 2)
+        sol = c.mv(B)
+        its = c.minres_solve(sol, None, 1e-13, 3000, distribute=True)
+        res[name] = dict(rhs=rhs.get(), ax=ax.get(), pj=pj.get(), dq=dq, sol=sol.get(), its=its,
+                         keep=(mpsi, x, rhs, xin, ax, pj, sol))
+    G, O = res["g"], res["o"]
+    assert rel(G["rhs"], O["rhs"]) < 1e-11
+    assert rel(G["ax"], O["ax"]) < 1e-13
+    assert rel(G["pj"], O["pj"]) < 1e-15
+    assert np.max(np.abs(G["dq"] - O["dq"])) < 1e-12
+    assert rel(G["sol"], O["sol"]) < 1e-8
+    assert np.all(np.abs(G["its"] - O["its"]) <= 3)
+
+
+@pytest.mark.parametrize("mode", [abi.GRADIENT_REFERENCE, abi.GRADIENT_EXACT])
+def test_get_force(glib, olib, mode):
+    _say("get_force", mode)
+    prob = small_problem(olib)
+    kw = dict(cheb_tol=1e-11, initial_tol_cheb=1e-11, gradient_mode=mode)
+    g = prob.context(glib, **kw)
+    o = prob.context(olib, **kw)
+    fg, lg = g.get_force(prob.vxc0)
+    fo, lo = o.get_force(prob.vxc0)
+    assert abs(lg - lo) / abs(lo) < 1e-9
+    assert rel(fg, fo) < 1e-7
+
+
+def test_mixed_precision_force(glib, olib):
+    _say("mixed_force")
+    prob = small_problem(olib)
+    kw = dict(cheb_tol=1e-10, initial_tol_cheb=1e-10)
+    gm = prob.context(glib, precision=abi.MIXED, **kw)
+    o = prob.context(olib, **kw)
+    fm, lm = gm.get_force(prob.vxc0)
+    fo, lo = o.get_force(prob.vxc0)
+    assert abs(lm - lo) / abs(lo) < 1e-7
+    assert rel(fm, fo) < 1e-6
+
+
+def test_bfgs_two_iterations(glib, olib):
+    _say("bfgs")
+    prob = small_problem(olib)
+    kw = dict(cheb_tol=1e-11, initial_tol_cheb=1e-11)
+    g = prob.context(glib, **kw)
+    o = prob.context(olib, **kw)
+    lg, gg, ng = g.bfgs_solve(max_iter=2)
+    lo, go, no = o.bfgs_solve(max_iter=2)
+    assert ng == no == 2
+    assert np.allclose(lg, lo, rtol=1e-7, atol=0)
+    assert np.allclose(gg, go, rtol=1e-6, atol=0)
+    assert lg[-1] < lg[0]
