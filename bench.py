@@ -1,0 +1,185 @@
+#!/usr/bin/env python3
+"""Seconds per BFGS iteration (eigensolve + adjoint + force) on one MI355X.
+
+Workload inverse_24c_p5: domain [-12,12]^3, 24^3 cells, degree 5 (121^3 =
+1,771,561 nodes), v_xc on the linear 49^3 child mesh, soft-Coulomb wells
+(Z=3 at the origin, Z=1 at (0,0,3)), 8 occupied + 4 buffer orbitals,
+Chebyshev degree 30, FP64 Cholesky-QR + Rayleigh-Ritz, adjoint MINRES and
+force, L-BFGS with the CP line search and one line-search step, so one
+iteration is two getForceVector evaluations.  Inputs are synthetic (the
+target density is the FP64 ground state of the wells alone; v_xc starts at a
+seeded sum of 8 Gaussians).
+
+The mixed-precision run (FP32-MFMA cell operator, defect-corrected FP32
+Chebyshev filter, FP32 inner MINRES with FP64 refinement, FP64 everything
+else) is timed over K iterations after W warm-up iterations, then the FP64
+run on the same problem.  Every library call returns only after its work on
+the GPU stream has completed, so host wall-clock brackets are synchronised.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+METRIC = "s per BFGS iter (eigen+adjoint+force), mixed vs FP64 on GPU, force err ≤1e-6"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def run_bfgs(lib, prob, precision, warmup, steps, tol_kw):
+    from invdft_amd import abi
+    ctx = prob.context(lib, precision=precision, **tol_kw)
+    t0 = time.perf_counter()
+    if warmup > 0:
+        ctx.bfgs_solve(ls_type=abi.LS_CP, max_iter=warmup)
+        ctx.set_initial_guess(ctx.get_solution())
+    st0 = ctx.stats()
+    log(f"  warmup {warmup} iterations: {time.perf_counter() - t0:.2f} s")
+    loss, gnorm, n = ctx.bfgs_solve(ls_type=abi.LS_CP, max_iter=steps)
+    th = ctx.bfgs_times_ms
+    st1 = ctx.stats()
+    timed_ms = th[n] - th[0]
+    evals = st1["n_evaluations"] - st0["n_evaluations"]
+    per_eval = {k: (st1[k] - st0[k]) / max(evals, 1)
+                for k in ("applies_fp64", "applies_fp32", "apply_cols_fp64", "apply_cols_fp32")}
+    log(f"  timed {n} iterations: {timed_ms:.1f} ms, loss {loss[0]:.4e} -> {loss[-1]:.4e}, "
+        f"last eval: passes={st1['cheb_passes']} minres_max={st1['minres_iters_max']} "
+        f"refinements={st1['refinements']} t_eig={st1['t_eigen_ms']:.1f} "
+        f"t_adj={st1['t_adjoint_ms']:.1f} t_force={st1['t_force_ms']:.1f} ms")
+    return ctx, dict(iters=n, timed_ms=timed_ms, s_per_iter=timed_ms / 1000.0 / n, loss=list(loss),
+                     gnorm=list(gnorm), per_eval=per_eval, last=st1, evals=evals)
+
+
+def cpu_baseline(prob, per_eval, evals_per_iter):
+    """Oracle timing sample: one FP64 H_e build and one 12-column apply on this
+    mesh, scaled by the GPU run's per-iteration operation counts."""
+    from oracle import oracle
+    from invdft_amd import abi
+    olib = oracle.load()
+    ctx = prob.context(olib)
+    s = ctx.sizes
+    nq = s.n_cells * s.quad_per_cell
+    t0 = time.perf_counter()
+    ctx.set_veff_quad(prob.pot_base_q)
+    t_build = time.perf_counter() - t0
+    B = prob.desc.n_eig
+    x = np.random.default_rng(0).standard_normal((s.n_nodes, B))
+    t0 = time.perf_counter()
+    ctx.apply_hamiltonian(x)
+    t_apply = time.perf_counter() - t0
+    cols = per_eval["apply_cols_fp64"] + per_eval["apply_cols_fp32"]
+    # per evaluation: H_e build, stored R_e build (same loop shape), column applies
+    t_eval = 2.0 * t_build + cols * t_apply / B
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    del nq
+    return {"value": round(t_eval * evals_per_iter, 3), "unit": "s/iter", "cores": cores,
+            "kind": "port",
+            "sample": (f"FP64 oracle on this mesh: H_e build {t_build:.2f} s + one {B}-column "
+                       f"Loewdin apply {t_apply:.2f} s, scaled to {cols:.0f} column-applies and "
+                       f"2 cell-matrix builds per evaluation, {evals_per_iter} evaluations per "
+                       f"iteration")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="inverse_24c_p5")
+    ap.add_argument("--cheb-tol", type=float, default=1e-8)
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (--gpus {args.gpus} requested); the multi-GPU "
+              "decomposition is a design note, not a measured path", file=sys.stderr)
+        sys.exit(2)
+
+    import invdft_amd
+    from invdft_amd import abi, synthetic
+
+    lib = invdft_amd.load()
+    cfg = dict(synthetic.CONFIGS[args.config])
+    log(f"config {args.config}: {cfg}")
+    t0 = time.perf_counter()
+    prob = synthetic.make_problem(lib, seed=2, target_tol=1e-10, **cfg)
+    log(f"setup (target density by FP64 eigensolve on the GPU): {time.perf_counter() - t0:.1f} s")
+    tol_kw = dict(cheb_tol=args.cheb_tol, initial_tol_cheb=args.cheb_tol)
+
+    log("mixed precision run")
+    cm, mixed = run_bfgs(lib, prob, abi.MIXED, args.warmup, args.steps, tol_kw)
+    fp64 = None
+    cf = None
+    if not args.no_fp64:
+        log("FP64 run")
+        cf, fp64 = run_bfgs(lib, prob, abi.FP64, args.warmup, args.steps, tol_kw)
+
+    # force error of the mixed path against FP64 at the same potential
+    force_err = None
+    if cf is not None:
+        x = cm.get_solution()
+        fm, lm = cm.get_force(x)
+        ff, lf = cf.get_force(x)
+        force_err = float(np.linalg.norm(fm - ff) / np.linalg.norm(ff))
+        log(f"force rel err mixed vs FP64: {force_err:.3e}; loss {lm:.6e} vs {lf:.6e}")
+
+    # roofline of the dominant kernel: the FP32 MFMA cell apply (8 colour launches)
+    s = cm.sizes
+    B = prob.desc.n_eig
+    ne = s.nodes_per_cell
+    ms32 = cm.profile_apply(B, abi.MIXED, reps=20)
+    bytes_apply = s.n_cells * ne * ne * 4 + 2 * s.n_nodes * B * 4
+    achieved = bytes_apply / (ms32 * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "k_cell_apply<float,5,1> (8 colour launches per apply)",
+            "ms_per_apply": round(ms32, 4), "bytes_per_apply": bytes_apply}
+    extra = {}
+    if cf is not None:
+        ms64 = cf.profile_apply(B, abi.FP64, reps=20)
+        b64 = s.n_cells * ne * ne * 8 + 2 * s.n_nodes * B * 8
+        extra["fp64_apply"] = {"ms_per_apply": round(ms64, 4),
+                               "achieved_GBs": round(b64 / (ms64 * 1e-3) / 1e9, 1)}
+    cpu = None
+    if not args.no_cpu:
+        log("CPU baseline sample (oracle)")
+        cpu = cpu_baseline(prob, mixed["per_eval"] if fp64 is None else fp64["per_eval"], 2)
+
+    out = {
+        "metric": METRIC,
+        "value": round(mixed["s_per_iter"], 4),
+        "unit": "s/iter",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(mixed["timed_ms"] / mixed["iters"], 2),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32-mfma+f64",
+        "data": "synthetic",
+        "config": {"workload": args.config, "n_cells": cfg["n_cells"], "degree": cfg["degree"],
+                   "nodes": int(s.n_nodes), "orbitals": B, "cheb_degree": prob.desc.cheb_degree,
+                   "cheb_tol": args.cheb_tol, "evals_per_iter": 2, "parallelism": "single-gpu"},
+        "fp64_s_per_iter": round(fp64["s_per_iter"], 4) if fp64 else None,
+        "speedup_mixed_vs_fp64": round(fp64["s_per_iter"] / mixed["s_per_iter"], 3) if fp64 else None,
+        "force_rel_err_mixed_vs_fp64": force_err,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "detail": {"mixed_per_eval": mixed["per_eval"], "fp64_per_eval": fp64["per_eval"] if fp64 else None,
+                   "mixed_loss": mixed["loss"], **extra},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

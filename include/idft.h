@@ -113,6 +113,10 @@ typedef struct idft_stats {
   double loss_unweighted, l1_density_error, int_rho_veff;
   int64_t n_evaluations;
   double spectrum_upper_bound;
+  int64_t applies_fp64;       /* Hamiltonian applications (cumulative)          */
+  int64_t applies_fp32;
+  int64_t apply_cols_fp64;    /* sum of block widths over those applications    */
+  int64_t apply_cols_fp32;
 } idft_stats;
 
 typedef struct idft_ctx idft_ctx;
@@ -239,12 +243,15 @@ int idft_minres_solve(idft_ctx* ctx, idft_mv* x, idft_mv* ndbc, double tol,
 /* ---- optimiser (host driver over contract 1) ------------------------------ */
 /* BFGSInverseDFTSolver::solve (BFGSInverseDFTSolver.cpp:465-627) over this
  * context's getForceVector.  loss_hist / gnorm_hist [max_iter+1] (may be
- * NULL) receive the loss and force norm at the start of each iteration;
- * n_done the iterations completed; the final potential is kept as the
- * solution (idft_get_solution). */
+ * NULL) receive the loss and force norm at the start of each iteration,
+ * time_hist_ms [max_iter+1] (may be NULL) the wall-clock time (ms, since the
+ * call began) at the start of each iteration and at the end; n_done the
+ * iterations completed; the final potential is kept as the solution
+ * (idft_get_solution). */
 int idft_bfgs_solve(idft_ctx* ctx, int32_t ls_type, int32_t max_iter, double tol,
                     double ls_tol, int32_t history, int32_t n_line_search,
-                    double* loss_hist, double* gnorm_hist, int32_t* n_done);
+                    double* loss_hist, double* gnorm_hist, double* time_hist_ms,
+                    int32_t* n_done);
 
 #ifdef __cplusplus
 }

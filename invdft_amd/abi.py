@@ -48,7 +48,9 @@ class Stats(C.Structure):
                 ("fermi_energy", C.c_double), ("cheb_tol_used", C.c_double),
                 ("adjoint_tol_used", C.c_double), ("loss_unweighted", C.c_double),
                 ("l1_density_error", C.c_double), ("int_rho_veff", C.c_double),
-                ("n_evaluations", C.c_int64), ("spectrum_upper_bound", C.c_double)]
+                ("n_evaluations", C.c_int64), ("spectrum_upper_bound", C.c_double),
+                ("applies_fp64", C.c_int64), ("applies_fp32", C.c_int64),
+                ("apply_cols_fp64", C.c_int64), ("apply_cols_fp32", C.c_int64)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -100,7 +102,7 @@ _PROTOS = {
     "idft_minres_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_int32,
                                     C.c_int32, _IP]),
     "idft_bfgs_solve": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_double, C.c_double,
-                                  C.c_int32, C.c_int32, _DP, _DP, _IP]),
+                                  C.c_int32, C.c_int32, _DP, _DP, _DP, _IP]),
 }
 
 SYMBOLS = tuple(_PROTOS)
@@ -376,7 +378,19 @@ class Context:
                    n_line_search=1):
         lh = np.zeros(max_iter + 1)
         gh = np.zeros(max_iter + 1)
+        th = np.zeros(max_iter + 1)
         nd = C.c_int32(0)
         self._check(self.lib.idft_bfgs_solve(self.h, ls_type, max_iter, tol, ls_tol, history,
-                                             n_line_search, _dp(lh), _dp(gh), C.byref(nd)))
+                                             n_line_search, _dp(lh), _dp(gh), _dp(th),
+                                             C.byref(nd)))
+        self.bfgs_times_ms = th[: nd.value + 1]
         return lh[: nd.value + 1], gh[: nd.value + 1], nd.value
+
+    def profile_apply(self, ncols, precision=FP64, reps=10):
+        """Device ms per Hamiltonian application (HIP library only)."""
+        out = np.zeros(1)
+        fn = self.lib.idft_profile_apply
+        fn.restype = C.c_int
+        fn.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, _DP]
+        self._check(fn(self.h, ncols, precision, reps, _dp(out)))
+        return float(out[0])

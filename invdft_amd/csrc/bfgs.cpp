@@ -3,6 +3,7 @@
 // inverse-Hessian product, CP / secant-loss / secant-force-norm line searches,
 // curvature test y.s > 0, history cap.  Vectors are host arrays of n_vxc
 // doubles (the reference keeps them as host deal.II vectors too).
+#include <chrono>
 #include <cmath>
 #include <deque>
 #include <string>
@@ -19,7 +20,11 @@ class Lbfgs {
   Lbfgs(idft_ctx* c, int64_t n, int history) : c_(c), n_(n), history_(history) {}
 
   int run(int lsType, int maxIter, double tol, double lsTol, int nLs, double* lossHist,
-          double* gnormHist, int32_t* nDone) {
+          double* gnormHist, double* timeHist, int32_t* nDone) {
+    const auto t0 = std::chrono::steady_clock::now();
+    auto elapsed = [&] {
+      return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    };
     x_.assign(n_, 0.0);
     int rc = idft_get_initial_guess(c_, x_.data());
     if (rc) return rc;
@@ -31,6 +36,7 @@ class Lbfgs {
     for (; iter < maxIter; ++iter) {
       if (iter == 0 && (rc = evaluate(x_, g_, L))) return rc;
       const double gn = std::sqrt(dot(g_, g_));
+      if (timeHist) timeHist[iter] = elapsed();
       if (lossHist) lossHist[iter] = L;
       if (gnormHist) gnormHist[iter] = gn;
       const bool converged = gn < tol;
@@ -68,6 +74,7 @@ class Lbfgs {
       if (lossHist) lossHist[iter] = L;
       if (gnormHist) gnormHist[iter] = std::sqrt(dot(g_, g_));
     }
+    if (timeHist) timeHist[iter] = elapsed();
     idft_set_solution(c_, x_.data());
     if (nDone) *nDone = iter;
     return IDFT_OK;
@@ -182,7 +189,8 @@ class Lbfgs {
 
 extern "C" int idft_bfgs_solve(idft_ctx* c, int32_t ls_type, int32_t max_iter, double tol,
                                double ls_tol, int32_t history, int32_t n_line_search,
-                               double* loss_hist, double* gnorm_hist, int32_t* n_done) {
+                               double* loss_hist, double* gnorm_hist, double* time_hist,
+                               int32_t* n_done) {
   if (!c || max_iter < 0 || history < 1 || n_line_search < 0) return IDFT_EINVAL;
   if (ls_type < IDFT_LS_SECANT_LOSS || ls_type > IDFT_LS_CP) return IDFT_EINVAL;
   idft_sizes sz;
@@ -190,7 +198,8 @@ extern "C" int idft_bfgs_solve(idft_ctx* c, int32_t ls_type, int32_t max_iter, d
   if (rc) return rc;
   try {
     Lbfgs solver(c, sz.n_vxc, history);
-    return solver.run(ls_type, max_iter, tol, ls_tol, n_line_search, loss_hist, gnorm_hist, n_done);
+    return solver.run(ls_type, max_iter, tol, ls_tol, n_line_search, loss_hist, gnorm_hist,
+                      time_hist, n_done);
   } catch (const std::exception&) {
     return IDFT_ENOMEM;
   }

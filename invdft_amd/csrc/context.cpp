@@ -100,10 +100,14 @@ struct idft_ctx {
   // Y += alpha S H S X (Loewdin Hamiltonian), FP64 or FP32 cell matrices.
   void happly64(const double* Xv, double* Yv, int b, double alpha) {
     launch_cell_apply<double>(g, H64.p, S.p, Xv, Yv, b, alpha, s);
+    st.applies_fp64++;
+    st.apply_cols_fp64 += b;
   }
   void happly32(const float* Xv, float* Yv, int b, double alpha) {
     if (!H32.p) throw Error(IDFT_ESTATE, "FP32 Hamiltonian not built (precision is FP64)");
     launch_cell_apply<float>(g, H32.p, S32.p, Xv, Yv, b, alpha, s);
+    st.applies_fp32++;
+    st.apply_cols_fp32 += b;
   }
   template <typename T>
   void happly(const T* Xv, T* Yv, int b, double alpha) {
@@ -1168,6 +1172,45 @@ int idft_minres_solve(idft_ctx* c, idft_mv* x, idft_mv* ndbc, double tol, int32_
     c->st.minres_iters_total = 0;
     c->st.refinements = 0;
     c->minres(x, tol, maxit, distribute, iters);
+  });
+}
+
+int idft_profile_apply(idft_ctx* c, int32_t ncols, int32_t prec, int32_t reps, double* ms) {
+  if (!c || !ms || ncols < 1 || ncols > kMaxCols || reps < 1) return IDFT_EINVAL;
+  if (!c->haveH) { c->err = "no Hamiltonian"; return IDFT_ESTATE; }
+  return guarded(c, [&] {
+    const size_t n = c->NB(ncols);
+    std::vector<double> h(n);
+    uint64_t sd = 12345;
+    for (auto& v : h) v = rand_pm1(sd);
+    DevBuf<double> xd, yd(n);
+    xd.upload(h.data(), n, c->s);
+    DevBuf<float> xf, yf;
+    if (prec == IDFT_MIXED) {
+      xf.resize(n); yf.resize(n);
+      launch_convert_d2f((int64_t)n, xf.p, xd.p, c->s);
+      launch_lincomb<float>(c->t.N, ncols, yf.p, 0.0, nullptr, 0.0, nullptr, nullptr, 0.0, nullptr, nullptr, c->s);
+    } else {
+      launch_lincomb<double>(c->t.N, ncols, yd.p, 0.0, nullptr, 0.0, nullptr, nullptr, 0.0, nullptr, nullptr, c->s);
+    }
+    auto once = [&] {
+      if (prec == IDFT_MIXED) c->happly32(xf.p, yf.p, ncols, 1e-3);
+      else c->happly64(xd.p, yd.p, ncols, 1e-3);
+    };
+    once();
+    once();
+    hipEvent_t e0, e1;
+    IDFT_HIP(hipEventCreate(&e0));
+    IDFT_HIP(hipEventCreate(&e1));
+    IDFT_HIP(hipEventRecord(e0, c->s));
+    for (int r = 0; r < reps; ++r) once();
+    IDFT_HIP(hipEventRecord(e1, c->s));
+    IDFT_HIP(hipEventSynchronize(e1));
+    float el = 0.0f;
+    IDFT_HIP(hipEventElapsedTime(&el, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms = el / reps;
   });
 }
 

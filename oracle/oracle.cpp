@@ -390,6 +390,7 @@ double fermi_level(const vector<double>& e, double ne, double T) {
 void apply_cells(idft_ctx* c, const vector<double>& Hc, const double* X, double* Y,
                  int B, bool lowdin) {
   const Mesh& m = c->m;
+  if (lowdin) { c->st.applies_fp64++; c->st.apply_cols_fp64 += B; }
   const int ne = m.ne;
   std::fill(Y, Y + (size_t)m.N * B, 0.0);
   for (int col = 0; col < 8; ++col) {
@@ -438,11 +439,13 @@ void build_hamiltonian(idft_ctx* c) {
         for (int l = 0; l < ne; ++l) T[(size_t)q * ne + l] = w * m.Nq[(size_t)q * ne + l];
       }
       double* H = &c->He[(size_t)e * ne * ne];
-      for (int a = 0; a < ne; ++a)
-        for (int b = 0; b < ne; ++b) {
-          double s = 0.0;
-          for (int q = 0; q < nq; ++q) s += m.Nq[(size_t)q * ne + a] * T[(size_t)q * ne + b];
-          H[(size_t)a * ne + b] = 0.5 * m.Ke[(size_t)a * ne + b] + s;
+      for (size_t i = 0; i < (size_t)ne * ne; ++i) H[i] = 0.5 * m.Ke[i];
+      for (int q = 0; q < nq; ++q)
+        for (int a = 0; a < ne; ++a) {
+          const double na = m.Nq[(size_t)q * ne + a];
+          const double* tq = &T[(size_t)q * ne];
+          double* ha = &H[(size_t)a * ne];
+          for (int b = 0; b < ne; ++b) ha[b] += na * tq[b];
         }
     }
   }
@@ -793,14 +796,18 @@ int adj_compute_rhs(idft_ctx* c, idft_mv* x, idft_mv* rhs) {
   // computeRMatrix (stored R_e, .cpp:309-366)
   vector<double> R((size_t)m.ncell * ne * ne, 0.0);
 #pragma omp parallel for schedule(static)
-  for (long e = 0; e < m.ncell; ++e)
-    for (int a = 0; a < ne; ++a)
-      for (int b = 0; b < ne; ++b) {
-        double s = 0.0;
-        for (int q = 0; q < nq; ++q)
-          s += m.Nq[(size_t)q * ne + a] * m.Nq[(size_t)q * ne + b] * A.uJxW[(size_t)e * nq + q];
-        R[((size_t)e * ne + a) * ne + b] = s;
+  for (long e = 0; e < m.ncell; ++e) {
+    double* Re = &R[(size_t)e * ne * ne];
+    for (int q = 0; q < nq; ++q) {
+      const double w = A.uJxW[(size_t)e * nq + q];
+      const double* nqv = &m.Nq[(size_t)q * ne];
+      for (int a = 0; a < ne; ++a) {
+        const double na = nqv[a] * w;
+        double* ra = &Re[(size_t)a * ne];
+        for (int b = 0; b < ne; ++b) ra[b] += na * nqv[b];
       }
+    }
+  }
   compute_mu(c, A.occ);
   // rhs = -2 Mu^T psiT  (xgemm at .cpp:722-725)
   rhs->v.assign((size_t)N * B, 0.0);
@@ -1626,7 +1633,8 @@ struct Bfgs {
 
 extern "C" int idft_bfgs_solve(idft_ctx* c, int32_t ls_type, int32_t max_iter, double tol,
                                double ls_tol, int32_t history, int32_t n_ls, double* loss_hist,
-                               double* gnorm_hist, int32_t* n_done) {
+                               double* gnorm_hist, double* time_hist, int32_t* n_done) {
+  const double tStart = now_ms();
   if (!c || max_iter < 0 || history < 1) return IDFT_EINVAL;
   if (ls_type < IDFT_LS_SECANT_LOSS || ls_type > IDFT_LS_CP) return fail(c, IDFT_EINVAL, "invalid line search type");
   Bfgs b;
@@ -1639,6 +1647,7 @@ extern "C" int idft_bfgs_solve(idft_ctx* c, int32_t ls_type, int32_t max_iter, d
   for (; iter < max_iter; ++iter) {
     if (iter == 0 && !b.eval(b.x, b.g, L)) return b.err;
     double gn = std::sqrt(b.dot(b.g, b.g));
+    if (time_hist) time_hist[iter] = now_ms() - tStart;
     if (loss_hist) loss_hist[iter] = L;
     if (gnorm_hist) gnorm_hist[iter] = gn;
     bool conv = gn < tol;
@@ -1661,6 +1670,7 @@ extern "C" int idft_bfgs_solve(idft_ctx* c, int32_t ls_type, int32_t max_iter, d
     loss_hist[iter] = L;
     if (gnorm_hist) gnorm_hist[iter] = std::sqrt(b.dot(b.g, b.g));
   }
+  if (time_hist) time_hist[iter] = now_ms() - tStart;
   c->potSol = b.x;
   if (n_done) *n_done = iter;
   return IDFT_OK;
