@@ -17,13 +17,14 @@ extern "C" {
 int idft_hip_init(void);
 
 /* Average device time (HIP events on the context's stream) of one
- * application of the Loewdin cell operator (the 8 colour passes of the MFMA
- * cell-apply kernel) on a block of ncols random columns, FP64 or FP32
- * (precision = IDFT_FP64 / IDFT_MIXED), over reps repetitions after 2 warm-up
- * applications.  No counterpart in the reference; used by the benchmark to
- * measure the dominant kernel live. */
+ * application of the Loewdin cell operator on a block of ncols random
+ * columns, FP64 or FP32 (precision = IDFT_FP64 / IDFT_MIXED), over reps
+ * repetitions after 2 warm-up applications: ms_per_apply for the whole
+ * operator (MFMA cell-product kernel + node-gather kernel), ms_gemm (may be
+ * NULL) for the MFMA cell-product kernel alone.  No counterpart in the
+ * reference; used by the benchmark to measure the dominant kernel live. */
 int idft_profile_apply(idft_ctx* ctx, int32_t ncols, int32_t precision, int32_t reps,
-                       double* ms_per_apply);
+                       double* ms_per_apply, double* ms_gemm);
 
 #ifdef __cplusplus
 }

@@ -387,10 +387,11 @@ class Context:
         return lh[: nd.value + 1], gh[: nd.value + 1], nd.value
 
     def profile_apply(self, ncols, precision=FP64, reps=10):
-        """Device ms per Hamiltonian application (HIP library only)."""
-        out = np.zeros(1)
+        """Device ms per Hamiltonian application and for its MFMA cell-product
+        kernel alone (HIP library only)."""
+        out = np.zeros(2)
         fn = self.lib.idft_profile_apply
         fn.restype = C.c_int
-        fn.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, _DP]
-        self._check(fn(self.h, ncols, precision, reps, _dp(out)))
-        return float(out[0])
+        fn.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, _DP, _DP]
+        self._check(fn(self.h, ncols, precision, reps, _dp(out[:1]), _dp(out[1:])))
+        return float(out[0]), float(out[1])

@@ -109,7 +109,7 @@ constexpr int kMaxCols = 80;  // widest block the cell apply handles in one pass
 // Per-column MINRES state slots: st[slot * B + col].
 enum MinresSlot {
   MS_BETA, MS_OLDB, MS_DBAR, MS_EPSLN, MS_PHIBAR, MS_CS, MS_SN, MS_ALFA, MS_DONE, MS_IT, MS_TOL,
-  MS_BETA1, MS_CV, MS_CR1, MS_CA2, MS_CWV, MS_CWW1, MS_CWW, MS_CX, MS_COUNT
+  MS_BETA1, MS_CV, MS_CVN, MS_CR1, MS_CA2, MS_CWV, MS_CWW1, MS_CWW, MS_CX, MS_COUNT
 };
 
 int64_t hamiltonian_tile_elems(int p);
@@ -117,10 +117,36 @@ int64_t hamiltonian_tile_elems(int p);
 // when h32 != nullptr, float).  jxw[q] is the (uniform) parent JxW.
 void launch_build_hamiltonian(const Geo& g, const DevTables& t, double hh, const double* veff,
                               const double* jxw, double* h64, float* h32, hipStream_t s);
-// Y[node] += alpha * S[node] (H_e (S X)_e)[node], all 8 colours, B <= kMaxCols.
+// Output combination of one operator application:
+//   Y = beta Y + alpha S[n] (sum_e P_e^T H_e P_e S X)[n] + a ca X + b cb Z + c cc W
+// (null coefficient arrays read as 1, null blocks as 0).
+// The operator input is xcol[j] Xin (xcol may be null); optionally V receives
+// that scaled input and dot_partial[blk][j] the block partials of V.Y
+// (gather_blocks(N, B) blocks).
 template <typename T>
-void launch_cell_apply(const Geo& g, const T* H, const T* S, const T* X, T* Y, int B,
-                       double alpha, hipStream_t s);
+struct ApplyEpilogue {
+  T* Y = nullptr;
+  double alpha = 1.0, beta = 0.0;
+  const T* X = nullptr; double a = 0.0; const double* ca = nullptr;
+  const T* Z = nullptr; double b = 0.0; const double* cb = nullptr;
+  const T* W = nullptr; double c = 0.0; const double* cc = nullptr;
+  const double* xcol = nullptr;
+  T* V = nullptr;
+  double* dot_partial = nullptr;
+};
+constexpr int kGatherBlocks = 2048;
+int gather_blocks(int64_t N, int B);
+// Cell operator in two launches: per-cell H_e (S xcol Xin)_e into the scratch
+// Yc (cell_scratch_elems(g, B) elements), then the node gather + epilogue.
+// Returns the number of gather blocks (rows of dot_partial).
+template <typename T>
+int launch_cell_apply(const Geo& g, const T* H, const T* S, const T* Xin, T* Yc, int B,
+                      const ApplyEpilogue<T>& ep, hipStream_t s);
+int64_t cell_scratch_elems(const Geo& g, int B);
+// Phase 1 alone (per-cell products into Yc), for profiling.
+template <typename T>
+void launch_cell_gemm(const Geo& g, const T* H, const T* S, const T* X, const double* xcol, T* Yc,
+                      int B, hipStream_t s);
 // Y = beta Y + ns[n] (a ca[j] X + b cb[j] Z); null pointers read as 1 (X/Z as 0).
 template <typename T>
 void launch_lincomb(int64_t N, int B, T* Y, double beta, const T* X, double a, const double* ca,
@@ -178,15 +204,20 @@ void launch_force(const Geo& g, const DevTables& t, const double* Psi, const dou
                   hipStream_t s);
 void launch_zero_rows(int64_t N, int B, double* X, const unsigned char* mask, hipStream_t s);
 // MINRES
-void launch_minres_start(int B, int nblk, const double* partial, double* st, hipStream_t s);
+void launch_minres_start(int B, int nblk, const double* partial, const double* negEig,
+                         double* st, hipStream_t s);
 void launch_minres_alpha(int B, int nblk, const double* partial, double* st, hipStream_t s);
 template <typename T>
 int launch_minres_lanczos(int64_t N, int B, T* y, const T* r2, T* rnew, const double* D,
                           const double* st, double* partial, hipStream_t s);
-void launch_minres_givens(int B, int nblk, const double* partial, double* st, hipStream_t s);
+void launch_minres_givens(int B, int nblk, const double* partial, const double* negEig,
+                          double* st, hipStream_t s);
 template <typename T>
 void launch_minres_update(int64_t N, int B, const T* v, T* w1, const T* w, double* x,
                           const double* st, hipStream_t s);
 void launch_copy_done(int B, const double* st, int* flags, hipStream_t s);
+template <typename T>
+void launch_minres_prep(int64_t N, int B, T* y, T* v, const T* r1, const double* negEig,
+                        const double* st, hipStream_t s);
 
 }  // namespace idft

@@ -68,8 +68,8 @@ struct idft_ctx {
   DevBuf<double> potBase, rhoT, weight, veff, rho, u;
   std::vector<double> hMdiag;
   // Hamiltonian
-  DevBuf<double> H64;
-  DevBuf<float> H32;
+  DevBuf<double> H64, Yc64;
+  DevBuf<float> H32, Yc32;
   bool haveH = false;
   // eigen state (Loewdin vectors) and work
   DevBuf<double> X, X2, HX, Xtmp, T2buf, rref, psiAll, force;
@@ -98,21 +98,37 @@ struct idft_ctx {
 
   // ---------------------------------------------------------------- apply
   // Y += alpha S H S X (Loewdin Hamiltonian), FP64 or FP32 cell matrices.
-  void happly64(const double* Xv, double* Yv, int b, double alpha) {
-    launch_cell_apply<double>(g, H64.p, S.p, Xv, Yv, b, alpha, s);
-    st.applies_fp64++;
-    st.apply_cols_fp64 += b;
-  }
-  void happly32(const float* Xv, float* Yv, int b, double alpha) {
-    if (!H32.p) throw Error(IDFT_ESTATE, "FP32 Hamiltonian not built (precision is FP64)");
-    launch_cell_apply<float>(g, H32.p, S32.p, Xv, Yv, b, alpha, s);
-    st.applies_fp32++;
-    st.apply_cols_fp32 += b;
-  }
+  // One application of the Loewdin cell operator with a fused epilogue
+  // (see ApplyEpilogue); returns the number of dot-partial rows.
   template <typename T>
-  void happly(const T* Xv, T* Yv, int b, double alpha) {
-    if constexpr (std::is_same<T, double>::value) happly64(Xv, Yv, b, alpha);
-    else happly32(Xv, Yv, b, alpha);
+  int apply(const T* Xin, int b, const ApplyEpilogue<T>& ep) {
+    if constexpr (std::is_same<T, double>::value) {
+      Yc64.resize(cell_scratch_elems(g, b));
+      st.applies_fp64++;
+      st.apply_cols_fp64 += b;
+      return launch_cell_apply<double>(g, H64.p, S.p, Xin, Yc64.p, b, ep, s);
+    } else {
+      if (!H32.p) throw Error(IDFT_ESTATE, "FP32 Hamiltonian not built (precision is FP64)");
+      Yc32.resize(cell_scratch_elems(g, b));
+      st.applies_fp32++;
+      st.apply_cols_fp32 += b;
+      return launch_cell_apply<float>(g, H32.p, S32.p, Xin, Yc32.p, b, ep, s);
+    }
+  }
+  // Y = alpha S H S X + beta Y
+  template <typename T>
+  void happly(const T* Xv, T* Yv, int b, double alpha, double beta = 0.0) {
+    ApplyEpilogue<T> ep;
+    ep.Y = Yv;
+    ep.alpha = alpha;
+    ep.beta = beta;
+    apply<T>(Xv, b, ep);
+  }
+  void happly64(const double* Xv, double* Yv, int b, double alpha, double beta = 0.0) {
+    happly<double>(Xv, Yv, b, alpha, beta);
+  }
+  void happly32(const float* Xv, float* Yv, int b, double alpha, double beta = 0.0) {
+    happly<float>(Xv, Yv, b, alpha, beta);
   }
 
   // ------------------------------------------------------- reductions
@@ -203,7 +219,6 @@ struct idft_ctx {
     std::vector<double> al, be;
     double beta = 0.0;
     for (int j = 0; j < k; ++j) {
-      launch_lincomb<T>(t.N, 1, w.p, 0.0, nullptr, 0.0, nullptr, nullptr, 0.0, nullptr, nullptr, s);
       happly<T>(v.p, w.p, 1, 1.0);
       const double a = coldot_t<T>(v.p, w.p, 1)[0];
       launch_lincomb<T>(t.N, 1, w.p, 1.0, v.p, -a, nullptr, vp.p, -beta, nullptr, nullptr, s);
@@ -249,13 +264,20 @@ struct idft_ctx {
     double sigma = e / (a0 - c);
     const double sigma1 = sigma, tau = 2.0 / sigma1;
     T *xk = x0, *yk = y0, *zk = z0;
-    launch_lincomb<T>(t.N, b, yk, 0.0, xk, -c * sigma1 / e, nullptr, nullptr, 0.0, nullptr, nullptr, s);
-    happly<T>(xk, yk, b, sigma1 / e);
+    {
+      ApplyEpilogue<T> ep;
+      ep.Y = yk; ep.alpha = sigma1 / e;
+      ep.X = xk; ep.a = -c * sigma1 / e;
+      apply<T>(xk, b, ep);
+    }
     for (int k = 2; k <= deg; ++k) {
       const double sigma2 = 1.0 / (tau - sigma);
       const double f = 2.0 * sigma2 / e;
-      launch_lincomb<T>(t.N, b, zk, 0.0, yk, -f * c, nullptr, xk, -sigma * sigma2, nullptr, nullptr, s);
-      happly<T>(yk, zk, b, f);
+      ApplyEpilogue<T> ep;
+      ep.Y = zk; ep.alpha = f;
+      ep.X = yk; ep.a = -f * c;
+      ep.Z = xk; ep.b = -sigma * sigma2;
+      apply<T>(yk, b, ep);
       T* tmp = xk; xk = yk; yk = zk; zk = tmp;
       sigma = sigma2;
     }
@@ -317,9 +339,12 @@ struct idft_ctx {
     launch_lincomb<float>(t.N, b, dk, 0.0, Rf.p, sigma1 / e, nullptr, nullptr, 0.0, nullptr, nullptr, s);
     for (int k = 1; k < deg; ++k) {
       const double s2 = 1.0 / (tau - sigma), f = 2.0 * s2 / e, gk = sigma * s2;
-      launch_lincomb3<float>(t.N, b, dn, 0.0, dk, -f * c, nullptr, dm, -gk, nullptr, Rf.p, f,
-                             ydev.p + (size_t)k * b, nullptr, s);
-      happly32(dk, dn, b, f);
+      ApplyEpilogue<float> ep;
+      ep.Y = dn; ep.alpha = f;
+      ep.X = dk; ep.a = -f * c;
+      ep.Z = dm; ep.b = -gk;
+      ep.W = Rf.p; ep.c = f; ep.cc = ydev.p + (size_t)k * b;
+      apply<float>(dk, b, ep);
       float* tmp = dm; dm = dk; dk = dn; dn = tmp;
       sigma = s2;
     }
@@ -358,7 +383,6 @@ struct idft_ctx {
 
   void rayleigh_ritz() {
     HX.resize(NB(B)); X2.resize(NB(B));
-    launch_lincomb<double>(t.N, B, HX.p, 0.0, nullptr, 0.0, nullptr, nullptr, 0.0, nullptr, nullptr, s);
     happly64(X.p, HX.p, B, 1.0);
     std::vector<double> Hp = gram(X.p, HX.p, B), Q;
     for (int i = 0; i < B; ++i)
@@ -534,8 +558,10 @@ struct idft_ctx {
   void vmult(double* Ax, double* x, int b) {
     if (!adj.ready) throw Error(IDFT_ESTATE, "vmult before updateInputPsi");
     zero_boundary(x, b);
-    launch_lincomb<double>(t.N, b, Ax, 0.0, x, 1.0, adj.dnegEig.p, nullptr, 0.0, nullptr, nullptr, s);
-    happly64(x, Ax, b, 1.0);
+    ApplyEpilogue<double> ep;
+    ep.Y = Ax; ep.alpha = 1.0;
+    ep.X = x; ep.a = 1.0; ep.ca = adj.dnegEig.p;
+    apply<double>(x, b, ep);
   }
 
   std::vector<double> dot_quad_wise(const double* v1, const double* v2, int b) {
@@ -607,9 +633,9 @@ struct idft_ctx {
     launch_lincomb<T>(N, b, W.y.p, 0.0, rhs, 1.0, nullptr, nullptr, 0.0, nullptr, Djac.p, s);
     launch_lincomb<T>(N, b, W.w1.p, 0.0, nullptr, 0.0, nullptr, nullptr, 0.0, nullptr, nullptr, s);
     launch_lincomb<T>(N, b, W.w.p, 0.0, nullptr, 0.0, nullptr, nullptr, 0.0, nullptr, nullptr, s);
-    partial.resize((size_t)1024 * b);
+    partial.resize((size_t)std::max(1024, kGatherBlocks) * b);
     int nb = launch_coldot_partial<T>(N, b, W.r1.p, W.y.p, nullptr, partial.p, s);
-    launch_minres_start(b, nb, partial.p, mst.p, s);
+    launch_minres_start(b, nb, partial.p, adj.dnegEig.p, mst.p, s);
     const int check = std::max(1, d.check_every);
     std::vector<int> flags(b);
     T *r1 = W.r1.p, *r2 = W.r2.p, *w1 = W.w1.p, *w = W.w.p;
@@ -622,15 +648,18 @@ struct idft_ctx {
         for (int f : flags) all = all && f;
         if (all) break;
       }
-      launch_lincomb<T>(N, b, W.v.p, 0.0, W.y.p, 1.0, mst.p + (size_t)MS_CV * b, nullptr, 0.0,
-                        nullptr, nullptr, s);
-      launch_lincomb<T>(N, b, W.y.p, 0.0, W.v.p, 1.0, adj.dnegEig.p, r1, 1.0,
-                        mst.p + (size_t)MS_CR1 * b, nullptr, s);
-      happly<T>(W.v.p, W.y.p, b, 1.0);
-      nb = launch_coldot_partial<T>(N, b, W.v.p, W.y.p, nullptr, partial.p, s);
+      // v = cv y;  y <- H~ v - eps v + cr1 r1;  partial dots of v.y  (one apply)
+      ApplyEpilogue<T> ep;
+      ep.Y = W.y.p; ep.alpha = 1.0;
+      ep.X = W.y.p; ep.a = 1.0; ep.ca = mst.p + (size_t)MS_CVN * b;
+      ep.Z = r1; ep.b = 1.0; ep.cb = mst.p + (size_t)MS_CR1 * b;
+      ep.xcol = mst.p + (size_t)MS_CV * b;
+      ep.V = W.v.p;
+      ep.dot_partial = partial.p;
+      nb = apply<T>(W.y.p, b, ep);
       launch_minres_alpha(b, nb, partial.p, mst.p, s);
       nb = launch_minres_lanczos<T>(N, b, W.y.p, r2, r1, Djac.p, mst.p, partial.p, s);
-      launch_minres_givens(b, nb, partial.p, mst.p, s);
+      launch_minres_givens(b, nb, partial.p, adj.dnegEig.p, mst.p, s);
       launch_minres_update<T>(N, b, W.v.p, w1, w, x, mst.p, s);
       std::swap(r1, r2);
       std::swap(w1, w);
@@ -670,8 +699,13 @@ struct idft_ctx {
       mw32.ensure(NB(b));
       int used = 0, sweeps = 0;
       while (used < maxit) {
-        launch_lincomb<double>(t.N, b, r, 0.0, xp, -1.0, adj.dnegEig.p, rhs.p, 1.0, nullptr, nullptr, s);
-        happly64(xp, r, b, -1.0);
+        {
+          ApplyEpilogue<double> ep;  // r = rhs - (H~ - eps) x
+          ep.Y = r; ep.alpha = -1.0;
+          ep.X = xp; ep.a = -1.0; ep.ca = adj.dnegEig.p;
+          ep.Z = rhs.p; ep.b = 1.0;
+          apply<double>(xp, b, ep);
+        }
         std::vector<double> rn = coldot(r, r, b, Djac.p);
         std::vector<double> tl(b);
         bool all = true;
@@ -1045,11 +1079,9 @@ int idft_apply_hamiltonian(idft_ctx* c, const double* x, double* y, int32_t ncol
     if (prec == IDFT_MIXED) {
       DevBuf<float> xf(n), yf(n);
       launch_convert_d2f((int64_t)n, xf.p, xd.p, c->s);
-      launch_lincomb<float>(c->t.N, ncols, yf.p, 0.0, nullptr, 0.0, nullptr, nullptr, 0.0, nullptr, nullptr, c->s);
       c->happly32(xf.p, yf.p, ncols, 1.0);
       launch_convert_f2d((int64_t)n, yd.p, yf.p, c->s);
     } else {
-      launch_lincomb<double>(c->t.N, ncols, yd.p, 0.0, nullptr, 0.0, nullptr, nullptr, 0.0, nullptr, nullptr, c->s);
       c->happly64(xd.p, yd.p, ncols, 1.0);
     }
     yd.download(y, n, c->s);
@@ -1175,7 +1207,8 @@ int idft_minres_solve(idft_ctx* c, idft_mv* x, idft_mv* ndbc, double tol, int32_
   });
 }
 
-int idft_profile_apply(idft_ctx* c, int32_t ncols, int32_t prec, int32_t reps, double* ms) {
+int idft_profile_apply(idft_ctx* c, int32_t ncols, int32_t prec, int32_t reps, double* ms,
+                       double* ms_gemm) {
   if (!c || !ms || ncols < 1 || ncols > kMaxCols || reps < 1) return IDFT_EINVAL;
   if (!c->haveH) { c->err = "no Hamiltonian"; return IDFT_ESTATE; }
   return guarded(c, [&] {
@@ -1186,31 +1219,37 @@ int idft_profile_apply(idft_ctx* c, int32_t ncols, int32_t prec, int32_t reps, d
     DevBuf<double> xd, yd(n);
     xd.upload(h.data(), n, c->s);
     DevBuf<float> xf, yf;
-    if (prec == IDFT_MIXED) {
+    const bool f32 = prec == IDFT_MIXED;
+    if (f32) {
       xf.resize(n); yf.resize(n);
       launch_convert_d2f((int64_t)n, xf.p, xd.p, c->s);
-      launch_lincomb<float>(c->t.N, ncols, yf.p, 0.0, nullptr, 0.0, nullptr, nullptr, 0.0, nullptr, nullptr, c->s);
-    } else {
-      launch_lincomb<double>(c->t.N, ncols, yd.p, 0.0, nullptr, 0.0, nullptr, nullptr, 0.0, nullptr, nullptr, c->s);
     }
     auto once = [&] {
-      if (prec == IDFT_MIXED) c->happly32(xf.p, yf.p, ncols, 1e-3);
-      else c->happly64(xd.p, yd.p, ncols, 1e-3);
+      if (f32) c->happly32(xf.p, yf.p, ncols, 1.0);
+      else c->happly64(xd.p, yd.p, ncols, 1.0);
     };
-    once();
-    once();
+    auto gemm = [&] {
+      if (f32) launch_cell_gemm<float>(c->g, c->H32.p, c->S32.p, xf.p, nullptr, c->Yc32.p, ncols, c->s);
+      else launch_cell_gemm<double>(c->g, c->H64.p, c->S.p, xd.p, nullptr, c->Yc64.p, ncols, c->s);
+    };
     hipEvent_t e0, e1;
     IDFT_HIP(hipEventCreate(&e0));
     IDFT_HIP(hipEventCreate(&e1));
-    IDFT_HIP(hipEventRecord(e0, c->s));
-    for (int r = 0; r < reps; ++r) once();
-    IDFT_HIP(hipEventRecord(e1, c->s));
-    IDFT_HIP(hipEventSynchronize(e1));
-    float el = 0.0f;
-    IDFT_HIP(hipEventElapsedTime(&el, e0, e1));
+    auto timed = [&](auto&& fn) {
+      fn();
+      fn();
+      IDFT_HIP(hipEventRecord(e0, c->s));
+      for (int r = 0; r < reps; ++r) fn();
+      IDFT_HIP(hipEventRecord(e1, c->s));
+      IDFT_HIP(hipEventSynchronize(e1));
+      float el = 0.0f;
+      IDFT_HIP(hipEventElapsedTime(&el, e0, e1));
+      return (double)el / reps;
+    };
+    *ms = timed(once);
+    if (ms_gemm) *ms_gemm = timed(gemm);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    *ms = el / reps;
   });
 }
 

@@ -17,6 +17,10 @@ namespace {
 
 constexpr int kThreads = 256;
 
+inline void check_u32(int64_t n) {
+  if (n >= (int64_t)0xffffffffLL) throw Error(IDFT_EINVAL, "multivector too large for 32-bit indexing");
+}
+
 __host__ __device__ constexpr int ne_of(int p) { return (p + 1) * (p + 1) * (p + 1); }
 __host__ __device__ constexpr int nep_of(int p) { return (ne_of(p) + 15) / 16 * 16; }
 
@@ -79,33 +83,36 @@ template <> struct Vec4<double> { typedef double4 type; };
 
 __host__ __device__ constexpr int ldb_of(int bt) { return (bt % 32 == 16) ? bt : bt + 16; }
 
-// One work-group = one cell of the given colour; 4 waves share the row tiles.
+// Phase 1 of the cell operator.  One work-group = one cell; the block (S X)_e
+// is gathered into LDS, the 4 waves share the 16-row tiles of H_e and stream
+// their tiles from HBM with one 16- (f32) or 32-byte (f64) load per lane per
+// four MFMA k-steps, prefetching KPF k-groups ahead.  Output: Yc[e][l][j] =
+// (H_e (S X)_e)[l][j] for j < B.  Work-groups are dealt to the 8 XCDs in
+// contiguous runs of cells.
 template <typename T, int P, int NCT>
-__global__ __launch_bounds__(kThreads) void k_cell_apply(Geo g, const T* __restrict__ H,
-                                                         const T* __restrict__ S,
-                                                         const T* __restrict__ X,
-                                                         T* __restrict__ Y, int B, T alpha,
-                                                         int colour, int64_t nw) {
+__global__ __launch_bounds__(kThreads) void k_cell_gemm(Geo g, const T* __restrict__ H,
+                                                        const T* __restrict__ S,
+                                                        const T* __restrict__ X,
+                                                        const double* __restrict__ xcol,
+                                                        T* __restrict__ Yc, int B) {
   constexpr int NE = ne_of(P), NEP = nep_of(P), RT = NEP / 16, BT = NCT * 16;
   constexpr int LDB = ldb_of(BT);
+  constexpr int KPF = 3;  // k-groups in flight per wave
   __shared__ T xs[NEP * LDB];
-  __shared__ T sc[NEP];
-  __shared__ int64_t nodes[NEP];
-  const int64_t e = colour_cell(g, colour, blockIdx.x, nw);
-  if (e < 0) return;
+  const int64_t nw = g.ncell;
+  const int64_t per = (nw + 7) / 8;
+  const int64_t e = (blockIdx.x % 8) * per + blockIdx.x / 8;
+  if (e >= nw) return;
   const int64_t cx = e % g.nc, cy = (e / g.nc) % g.nc, cz = e / ((int64_t)g.nc * g.nc);
   const int tid = threadIdx.x;
-  for (int l = tid; l < NEP; l += kThreads) {
-    int64_t n = l < NE ? cell_node(g, cx, cy, cz, l) : 0;
-    nodes[l] = n;
-    sc[l] = l < NE ? S[n] : T(0);
-  }
-  __syncthreads();
-  // gather S X into LDS (zero padding rows / columns)
   for (int i = tid; i < NEP * BT; i += kThreads) {
     const int l = i / BT, j = i % BT;
     T v = T(0);
-    if (l < NE && j < B) v = sc[l] * X[nodes[l] * B + j];
+    if (l < NE && j < B) {
+      const int64_t n = cell_node(g, cx, cy, cz, l);
+      v = S[n] * X[n * B + j];
+      if (xcol) v *= (T)xcol[j];
+    }
     xs[l * LDB + j] = v;
   }
   __syncthreads();
@@ -114,27 +121,35 @@ __global__ __launch_bounds__(kThreads) void k_cell_apply(Geo g, const T* __restr
   typedef typename Mfma<T>::acc_t acc_t;
   typedef typename Vec4<T>::type vec_t;
   const vec_t* Ht = reinterpret_cast<const vec_t*>(H + (size_t)e * NEP * NEP);
+  T* yc = Yc + (size_t)e * NE * B;
   for (int rt = wave; rt < RT; rt += 4) {
     acc_t acc[NCT];
 #pragma unroll
     for (int ct = 0; ct < NCT; ++ct) acc[ct] = acc_t{0, 0, 0, 0};
     const vec_t* row = Ht + (size_t)rt * RT * 64 + lane;
-    vec_t a = row[0];
+    vec_t buf[KPF];
+#pragma unroll
+    for (int k = 0; k < KPF; ++k) buf[k] = k < RT ? row[k * 64] : vec_t{};
 #pragma unroll 1
-    for (int kg = 0; kg < RT; ++kg) {
-      vec_t an = a;
-      if (kg + 1 < RT) an = row[(kg + 1) * 64];
-      const T* xk = xs + (kg * 16 + lk) * LDB + lr;
-      const T av[4] = {a.x, a.y, a.z, a.w};
+    for (int kg0 = 0; kg0 < RT; kg0 += KPF) {
 #pragma unroll
-      for (int s = 0; s < 4; ++s) {
+      for (int k = 0; k < KPF; ++k) {
+        const int kg = kg0 + k;
+        if (kg < RT) {
+          const vec_t a = buf[k];
+          if (kg + KPF < RT) buf[k] = row[(kg + KPF) * 64];
+          const T* xk = xs + (kg * 16 + lk) * LDB + lr;
+          const T av[4] = {a.x, a.y, a.z, a.w};
 #pragma unroll
-        for (int ct = 0; ct < NCT; ++ct)
-          acc[ct] = Mfma<T>::run(av[s], xk[s * 4 * LDB + ct * 16], acc[ct]);
+          for (int s = 0; s < 4; ++s) {
+#pragma unroll
+            for (int ct = 0; ct < NCT; ++ct)
+              acc[ct] = Mfma<T>::run(av[s], xk[s * 4 * LDB + ct * 16], acc[ct]);
+          }
+        }
       }
-      a = an;
     }
-    // scatter: lane holds rows rt*16 + Mfma<T>::row(lk, i), column ct*16 + lr
+    // lane holds rows rt*16 + Mfma<T>::row(lk, i), column ct*16 + lr
 #pragma unroll
     for (int ct = 0; ct < NCT; ++ct) {
       const int col = ct * 16 + lr;
@@ -142,11 +157,85 @@ __global__ __launch_bounds__(kThreads) void k_cell_apply(Geo g, const T* __restr
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int r = rt * 16 + Mfma<T>::row(lk, i);
-        if (r < NE) {
-          T* yp = Y + nodes[r] * B + col;
-          *yp += alpha * sc[r] * acc[ct][i];
-        }
+        if (r < NE) yc[(size_t)r * B + col] = acc[ct][i];
       }
+    }
+  }
+}
+
+// Phase 2: every node sums its contributions from the (at most 8) cells that
+// share it -- found arithmetically on the structured mesh, so no atomics and
+// no colouring -- and combines them with up to three other blocks:
+//   Y = beta Y + alpha S[n] sum + a ca[j] X + b cb[j] Z + c cc[j] W.
+// Optionally V = xcol[j] Xin (the scaled operator input) and per-column block
+// partials of V.Y (FP64).  Thread t owns column t % B (as in k_coldot).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_cell_gather(
+    Geo g, uint32_t B, const T* __restrict__ Yc, const T* __restrict__ S, T* __restrict__ Y,
+    double alpha, double beta, const T* __restrict__ X, double a, const double* __restrict__ ca,
+    const T* __restrict__ Z, double b, const double* __restrict__ cb, const T* __restrict__ W,
+    double c, const double* __restrict__ cc, const T* __restrict__ Xin,
+    const double* __restrict__ xcol, T* __restrict__ V, double* __restrict__ partial) {
+  __shared__ double red[kThreads];
+  const uint32_t N1 = g.N1, p = g.p, n1 = g.n1, nc = g.nc;
+  const uint32_t ne = n1 * n1 * n1;
+  const uint32_t N = (uint32_t)g.N;
+  const uint32_t tid = threadIdx.x, per = kThreads / B;
+  const uint32_t j = tid % B, off = tid / B;
+  double dot = 0.0;
+  if (off < per) {
+    const double caj = X ? a * (ca ? ca[j] : 1.0) : 0.0;
+    const double cbj = Z ? b * (cb ? cb[j] : 1.0) : 0.0;
+    const double ccj = W ? c * (cc ? cc[j] : 1.0) : 0.0;
+    const double xs = xcol ? xcol[j] : 1.0;
+    for (uint32_t n = blockIdx.x * per + off; n < N; n += gridDim.x * per) {
+      const uint32_t i = n * B + j;
+      const uint32_t ix = n % N1, iy = (n / N1) % N1, iz = n / (N1 * N1);
+      uint32_t cxs[2], axs[2], cys[2], ays[2], czs[2], azs[2];
+      int nx = 0, ny = 0, nz = 0;
+      const uint32_t qx = ix / p, rx = ix - qx * p, qy = iy / p, ry = iy - qy * p,
+                     qz = iz / p, rz = iz - qz * p;
+      if (rx == 0) {
+        if (qx > 0) { cxs[nx] = qx - 1; axs[nx++] = p; }
+        if (qx < nc) { cxs[nx] = qx; axs[nx++] = 0; }
+      } else { cxs[nx] = qx; axs[nx++] = rx; }
+      if (ry == 0) {
+        if (qy > 0) { cys[ny] = qy - 1; ays[ny++] = p; }
+        if (qy < nc) { cys[ny] = qy; ays[ny++] = 0; }
+      } else { cys[ny] = qy; ays[ny++] = ry; }
+      if (rz == 0) {
+        if (qz > 0) { czs[nz] = qz - 1; azs[nz++] = p; }
+        if (qz < nc) { czs[nz] = qz; azs[nz++] = 0; }
+      } else { czs[nz] = qz; azs[nz++] = rz; }
+      double sum = 0.0;
+      for (int kz = 0; kz < nz; ++kz)
+        for (int ky = 0; ky < ny; ++ky)
+          for (int kx = 0; kx < nx; ++kx) {
+            const uint64_t e = ((uint64_t)czs[kz] * nc + cys[ky]) * nc + cxs[kx];
+            const uint32_t l = (azs[kz] * n1 + ays[ky]) * n1 + axs[kx];
+            sum += (double)Yc[(e * ne + l) * B + j];
+          }
+      double v = alpha * (double)S[n] * sum;
+      if (X) v += caj * (double)X[i];
+      if (Z) v += cbj * (double)Z[i];
+      if (W) v += ccj * (double)W[i];
+      if (beta != 0.0) v += beta * (double)Y[i];
+      const T yo = (T)v;
+      if (V) {
+        const T vv = (T)(xs * (double)Xin[i]);
+        V[i] = vv;
+        dot += (double)vv * (double)yo;
+      }
+      Y[i] = yo;
+    }
+  }
+  if (partial) {
+    red[tid] = dot;
+    __syncthreads();
+    if (tid < B) {
+      double t = 0.0;
+      for (uint32_t k = 0; k < per; ++k) t += red[k * B + tid];
+      partial[(int64_t)blockIdx.x * B + tid] = t;
     }
   }
 }
@@ -534,21 +623,20 @@ __global__ __launch_bounds__(kThreads) void k_force(Geo g, const double* __restr
 
 // --------------------------------------------------------- vector kernels
 // Y = beta Y + ns[n] (a ca[j] X + b cb[j] Z + c cc[j] W); null coefficient
-// pointers read as 1, null operands as 0.
+// pointers read as 1, null operands as 0.  N * B < 2^32 (checked by the host).
 template <typename T>
-__global__ void k_lincomb(int64_t N, int B, T* __restrict__ Y, double beta,
+__global__ void k_lincomb(uint32_t NB, uint32_t B, T* __restrict__ Y, double beta,
                           const T* __restrict__ X, double a, const double* __restrict__ ca,
                           const T* __restrict__ Z, double b, const double* __restrict__ cb,
                           const T* __restrict__ W, double c, const double* __restrict__ cc,
                           const double* __restrict__ ns) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N * B;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const int j = (int)(i % B);
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < NB; i += gridDim.x * blockDim.x) {
+    const uint32_t n = i / B, j = i - n * B;
     double v = 0.0;
     if (X) v = a * (ca ? ca[j] : 1.0) * (double)X[i];
     if (Z) v += b * (cb ? cb[j] : 1.0) * (double)Z[i];
     if (W) v += c * (cc ? cc[j] : 1.0) * (double)W[i];
-    if (ns) v *= ns[i / B];
+    if (ns) v *= ns[n];
     if (beta != 0.0) v += beta * (double)Y[i];
     Y[i] = (T)v;
   }
@@ -601,8 +689,9 @@ __global__ void k_sum_partials(int nblk, int B, const double* __restrict__ parti
   out[j] = s;
 }
 
-// Gram partials: each block accumulates G[i][j] over a strided set of node
-// chunks; rows of X and Y are staged in LDS.
+// Gram partials: block (bx, by) accumulates the entries ij in
+// [by * 256, by * 256 + 256) of G over a strided set of 32-node chunks whose
+// rows of X and Y are staged in LDS.
 constexpr int kGramChunk = 32;
 __global__ __launch_bounds__(kThreads) void k_gram(int64_t N, int B, const double* __restrict__ X,
                                                    const double* __restrict__ Y,
@@ -612,47 +701,37 @@ __global__ __launch_bounds__(kThreads) void k_gram(int64_t N, int B, const doubl
   double* ys = lds + kGramChunk * B;
   const int tid = threadIdx.x;
   const int BB = B * B;
-  double acc[32];
-  const int nmine = (BB + kThreads - 1) / kThreads;  // <= 32 for B <= 90
-  for (int k = 0; k < nmine; ++k) acc[k] = 0.0;
+  const int ij = blockIdx.y * kThreads + tid;
+  const int i = ij / B, j = ij % B;
+  double acc = 0.0;
   for (int64_t n0 = (int64_t)blockIdx.x * kGramChunk; n0 < N; n0 += (int64_t)gridDim.x * kGramChunk) {
     const int cnt = (int)((N - n0) < kGramChunk ? (N - n0) : kGramChunk);
     __syncthreads();
-    for (int i = tid; i < kGramChunk * B; i += kThreads) {
-      const int r = i / B;
-      xs[i] = r < cnt ? X[n0 * B + i] : 0.0;
-      ys[i] = r < cnt ? Y[n0 * B + i] : 0.0;
+    for (int t = tid; t < kGramChunk * B; t += kThreads) {
+      const int r = t / B;
+      xs[t] = r < cnt ? X[n0 * B + t] : 0.0;
+      ys[t] = r < cnt ? Y[n0 * B + t] : 0.0;
     }
     __syncthreads();
-    for (int k = 0; k < nmine; ++k) {
-      const int ij = tid + k * kThreads;
-      if (ij >= BB) break;
-      const int i = ij / B, j = ij % B;
-      double s = 0.0;
-      for (int r = 0; r < cnt; ++r) s += xs[r * B + i] * ys[r * B + j];
-      acc[k] += s;
-    }
+    if (ij < BB)
+      for (int r = 0; r < cnt; ++r) acc += xs[r * B + i] * ys[r * B + j];
   }
-  for (int k = 0; k < nmine; ++k) {
-    const int ij = tid + k * kThreads;
-    if (ij < BB) partial[(int64_t)blockIdx.x * BB + ij] = acc[k];
-  }
+  if (ij < BB) partial[(int64_t)blockIdx.x * BB + ij] = acc;
 }
 
 // Y = beta Y + ns[n] (X Q)[n]  (X != Y)
-__global__ void k_right_mult(int64_t N, int B, const double* __restrict__ X,
+__global__ void k_right_mult(uint32_t NB, uint32_t B, const double* __restrict__ X,
                              const double* __restrict__ Q, double* __restrict__ Y,
                              const double* __restrict__ ns, double beta) {
   extern __shared__ double qs[];
-  for (int i = threadIdx.x; i < B * B; i += blockDim.x) qs[i] = Q[i];
+  for (uint32_t i = threadIdx.x; i < B * B; i += blockDim.x) qs[i] = Q[i];
   __syncthreads();
-  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < N * B;
-       t += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t n = t / B;
-    const int j = (int)(t % B);
-    const double* xr = X + n * B;
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < NB; t += gridDim.x * blockDim.x) {
+    const uint32_t n = t / B;
+    const uint32_t j = t - n * B;
+    const double* xr = X + (size_t)n * B;
     double s = 0.0;
-    for (int i = 0; i < B; ++i) s += xr[i] * qs[i * B + j];
+    for (uint32_t i = 0; i < B; ++i) s += xr[i] * qs[i * B + j];
     if (ns) s *= ns[n];
     Y[t] = beta != 0.0 ? beta * Y[t] + s : s;
   }
@@ -710,10 +789,9 @@ __global__ void k_loss(int64_t nQ, int nq, const double* __restrict__ rho,
   if (tid < 4) partial[blockIdx.x * 4 + tid] = red[tid][0];
 }
 
-__global__ void k_zero_rows(int64_t N, int B, double* __restrict__ X,
+__global__ void k_zero_rows(uint32_t NB, uint32_t B, double* __restrict__ X,
                             const unsigned char* __restrict__ mask) {
-  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < N * B;
-       t += (int64_t)gridDim.x * blockDim.x)
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < NB; t += gridDim.x * blockDim.x)
     if (mask[t / B]) X[t] = 0.0;
 }
 
@@ -721,7 +799,7 @@ __global__ void k_zero_rows(int64_t N, int B, double* __restrict__ X,
 // ------------------------------------------------------------- MINRES
 // Per-column state st[k * B + j] (see MinresSlot); one thread per column.
 __global__ void k_minres_start(int B, int nblk, const double* __restrict__ partial,
-                               double* __restrict__ st) {
+                               const double* __restrict__ negEig, double* __restrict__ st) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= B) return;
   double s = 0.0;
@@ -739,6 +817,7 @@ __global__ void k_minres_start(int B, int nblk, const double* __restrict__ parti
   const bool done = !(beta > st[MS_TOL * B + j]);
   st[MS_DONE * B + j] = done ? 1.0 : 0.0;
   st[MS_CV * B + j] = done ? 0.0 : 1.0 / beta;
+  st[MS_CVN * B + j] = st[MS_CV * B + j] * negEig[j];
   st[MS_CR1 * B + j] = 0.0;
 }
 
@@ -788,13 +867,14 @@ __global__ __launch_bounds__(kThreads) void k_minres_lanczos(int64_t N, int B, T
 }
 
 __global__ void k_minres_givens(int B, int nblk, const double* __restrict__ partial,
-                                double* __restrict__ st) {
+                                const double* __restrict__ negEig, double* __restrict__ st) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= B) return;
   double* S = st;
   if (S[MS_DONE * B + j] != 0.0) {
     S[MS_CWV * B + j] = 0.0; S[MS_CWW1 * B + j] = 0.0; S[MS_CWW * B + j] = 0.0;
     S[MS_CX * B + j] = 0.0; S[MS_CV * B + j] = 0.0; S[MS_CR1 * B + j] = 0.0;
+    S[MS_CVN * B + j] = 0.0;
     return;
   }
   double s = 0.0;
@@ -827,21 +907,34 @@ __global__ void k_minres_givens(int B, int nblk, const double* __restrict__ part
   const bool done = !(phibar > S[MS_TOL * B + j]) || beta == 0.0;
   S[MS_DONE * B + j] = done ? 1.0 : 0.0;
   S[MS_CV * B + j] = done ? 0.0 : 1.0 / beta;
+  S[MS_CVN * B + j] = S[MS_CV * B + j] * negEig[j];
   S[MS_CR1 * B + j] = done ? 0.0 : -beta / oldb;
 }
 
 // w_new = cwv v + cww1 w1 + cww w  (written over w1);  x += cx w_new
 template <typename T>
-__global__ void k_minres_update(int64_t N, int B, const T* __restrict__ v, T* __restrict__ w1,
-                                const T* __restrict__ w, double* __restrict__ x,
-                                const double* __restrict__ st) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N * B;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const int j = (int)(i % B);
+__global__ void k_minres_update(uint32_t NB, uint32_t B, const T* __restrict__ v,
+                                T* __restrict__ w1, const T* __restrict__ w,
+                                double* __restrict__ x, const double* __restrict__ st) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < NB; i += gridDim.x * blockDim.x) {
+    const uint32_t j = i % B;
     const double wn = st[MS_CWV * B + j] * (double)v[i] + st[MS_CWW1 * B + j] * (double)w1[i] +
                       st[MS_CWW * B + j] * (double)w[i];
     w1[i] = (T)wn;
     x[i] += st[MS_CX * B + j] * wn;
+  }
+}
+
+// v = cv y;  y = -eps v + cr1 r1  (the operator application then adds H~ v)
+template <typename T>
+__global__ void k_minres_prep(uint32_t NB, uint32_t B, T* __restrict__ y, T* __restrict__ v,
+                              const T* __restrict__ r1, const double* __restrict__ negEig,
+                              const double* __restrict__ st) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < NB; i += gridDim.x * blockDim.x) {
+    const uint32_t j = i % B;
+    const T vv = (T)(st[MS_CV * B + j] * (double)y[i]);
+    v[i] = vv;
+    y[i] = (T)(negEig[j] * (double)vv + st[MS_CR1 * B + j] * (double)r1[i]);
   }
 }
 
@@ -919,53 +1012,87 @@ void launch_build_hamiltonian(const Geo& g, const DevTables& t, double hh, const
 }
 
 template <typename T, int P>
-static void cell_apply_p(const Geo& g, const T* H, const T* S, const T* X, T* Y, int B,
-                         double alpha, hipStream_t s) {
+static void cell_gemm_p(const Geo& g, const T* H, const T* S, const T* X, const double* xcol,
+                        T* Yc, int B, hipStream_t s) {
   const int nct = (B + 15) / 16;
-  for (int colour = 0; colour < 8; ++colour) {
-    const int64_t nw = colour_count(g, colour);
-    if (nw == 0) continue;
-    dim3 grid(colour_grid(nw)), block(kThreads);
-    switch (nct) {
-      case 1: k_cell_apply<T, P, 1><<<grid, block, 0, s>>>(g, H, S, X, Y, B, (T)alpha, colour, nw); break;
-      case 2: k_cell_apply<T, P, 2><<<grid, block, 0, s>>>(g, H, S, X, Y, B, (T)alpha, colour, nw); break;
-      case 3: k_cell_apply<T, P, 3><<<grid, block, 0, s>>>(g, H, S, X, Y, B, (T)alpha, colour, nw); break;
-      case 4: k_cell_apply<T, P, 4><<<grid, block, 0, s>>>(g, H, S, X, Y, B, (T)alpha, colour, nw); break;
-      case 5: k_cell_apply<T, P, 5><<<grid, block, 0, s>>>(g, H, S, X, Y, B, (T)alpha, colour, nw); break;
-      default: throw Error(IDFT_EINVAL, "block wider than 80 columns");
-    }
+  dim3 grid((unsigned)(8 * ((g.ncell + 7) / 8))), block(kThreads);
+  switch (nct) {
+    case 1: k_cell_gemm<T, P, 1><<<grid, block, 0, s>>>(g, H, S, X, xcol, Yc, B); break;
+    case 2: k_cell_gemm<T, P, 2><<<grid, block, 0, s>>>(g, H, S, X, xcol, Yc, B); break;
+    case 3: k_cell_gemm<T, P, 3><<<grid, block, 0, s>>>(g, H, S, X, xcol, Yc, B); break;
+    case 4: k_cell_gemm<T, P, 4><<<grid, block, 0, s>>>(g, H, S, X, xcol, Yc, B); break;
+    case 5: k_cell_gemm<T, P, 5><<<grid, block, 0, s>>>(g, H, S, X, xcol, Yc, B); break;
+    default: throw Error(IDFT_EINVAL, "block wider than 80 columns");
   }
-  IDFT_HIP(hipGetLastError());
+}
+
+int64_t cell_scratch_elems(const Geo& g, int B) { return g.ncell * (int64_t)ne_of(g.p) * B; }
+
+int gather_blocks(int64_t N, int B) {
+  const int per = kThreads / B;
+  int64_t g = (N + per - 1) / per;
+  if (g > kGatherBlocks) g = kGatherBlocks;
+  return (int)(g < 1 ? 1 : g);
 }
 
 template <typename T>
-void launch_cell_apply(const Geo& g, const T* H, const T* S, const T* X, T* Y, int B,
-                       double alpha, hipStream_t s) {
+void launch_cell_gemm(const Geo& g, const T* H, const T* S, const T* X, const double* xcol, T* Yc,
+                      int B, hipStream_t s) {
+  if (B < 1 || B > kMaxCols) throw Error(IDFT_EINVAL, "block width out of range");
   switch (g.p) {
-    case 2: cell_apply_p<T, 2>(g, H, S, X, Y, B, alpha, s); break;
-    case 3: cell_apply_p<T, 3>(g, H, S, X, Y, B, alpha, s); break;
-    case 4: cell_apply_p<T, 4>(g, H, S, X, Y, B, alpha, s); break;
-    case 5: cell_apply_p<T, 5>(g, H, S, X, Y, B, alpha, s); break;
+    case 2: cell_gemm_p<T, 2>(g, H, S, X, xcol, Yc, B, s); break;
+    case 3: cell_gemm_p<T, 3>(g, H, S, X, xcol, Yc, B, s); break;
+    case 4: cell_gemm_p<T, 4>(g, H, S, X, xcol, Yc, B, s); break;
+    case 5: cell_gemm_p<T, 5>(g, H, S, X, xcol, Yc, B, s); break;
     default: throw Error(IDFT_EINVAL, "degree not supported by the HIP kernels (2..5)");
   }
+  IDFT_HIP(hipGetLastError());
 }
-template void launch_cell_apply<float>(const Geo&, const float*, const float*, const float*,
-                                       float*, int, double, hipStream_t);
-template void launch_cell_apply<double>(const Geo&, const double*, const double*, const double*,
-                                        double*, int, double, hipStream_t);
+template void launch_cell_gemm<float>(const Geo&, const float*, const float*, const float*,
+                                      const double*, float*, int, hipStream_t);
+template void launch_cell_gemm<double>(const Geo&, const double*, const double*, const double*,
+                                       const double*, double*, int, hipStream_t);
+
+template <typename T>
+int launch_cell_apply(const Geo& g, const T* H, const T* S, const T* X, T* Yc, int B,
+                      const ApplyEpilogue<T>& ep, hipStream_t s) {
+  if (B < 1 || B > kMaxCols) throw Error(IDFT_EINVAL, "block width out of range");
+  switch (g.p) {
+    case 2: cell_gemm_p<T, 2>(g, H, S, X, ep.xcol, Yc, B, s); break;
+    case 3: cell_gemm_p<T, 3>(g, H, S, X, ep.xcol, Yc, B, s); break;
+    case 4: cell_gemm_p<T, 4>(g, H, S, X, ep.xcol, Yc, B, s); break;
+    case 5: cell_gemm_p<T, 5>(g, H, S, X, ep.xcol, Yc, B, s); break;
+    default: throw Error(IDFT_EINVAL, "degree not supported by the HIP kernels (2..5)");
+  }
+  IDFT_HIP(hipGetLastError());
+  check_u32(g.N * B);
+  const int nb = gather_blocks(g.N, B);
+  k_cell_gather<T><<<nb, kThreads, 0, s>>>(g, (uint32_t)B, Yc, S, ep.Y, ep.alpha, ep.beta, ep.X,
+                                           ep.a, ep.ca, ep.Z, ep.b, ep.cb, ep.W, ep.c, ep.cc, X,
+                                           ep.xcol, ep.V, ep.dot_partial);
+  IDFT_HIP(hipGetLastError());
+  return nb;
+}
+template int launch_cell_apply<float>(const Geo&, const float*, const float*, const float*, float*,
+                                      int, const ApplyEpilogue<float>&, hipStream_t);
+template int launch_cell_apply<double>(const Geo&, const double*, const double*, const double*,
+                                       double*, int, const ApplyEpilogue<double>&, hipStream_t);
 
 template <typename T>
 void launch_lincomb(int64_t N, int B, T* Y, double beta, const T* X, double a, const double* ca,
                     const T* Z, double b, const double* cb, const double* ns, hipStream_t s) {
-  k_lincomb<T><<<grid_for(N * B), kThreads, 0, s>>>(N, B, Y, beta, X, a, ca, Z, b, cb, nullptr,
-                                                    0.0, nullptr, ns);
+  check_u32(N * B);
+  k_lincomb<T><<<grid_for(N * B), kThreads, 0, s>>>((uint32_t)(N * B), (uint32_t)B, Y, beta, X, a,
+                                                    ca, Z, b, cb, nullptr, 0.0, nullptr, ns);
   IDFT_HIP(hipGetLastError());
 }
 template <typename T>
 void launch_lincomb3(int64_t N, int B, T* Y, double beta, const T* X, double a, const double* ca,
                      const T* Z, double b, const double* cb, const T* W, double c,
                      const double* cc, const double* ns, hipStream_t s) {
-  k_lincomb<T><<<grid_for(N * B), kThreads, 0, s>>>(N, B, Y, beta, X, a, ca, Z, b, cb, W, c, cc, ns);
+  check_u32(N * B);
+  k_lincomb<T><<<grid_for(N * B), kThreads, 0, s>>>((uint32_t)(N * B), (uint32_t)B, Y, beta, X, a,
+                                                    ca, Z, b, cb, W, c, cc, ns);
   IDFT_HIP(hipGetLastError());
 }
 template void launch_lincomb<float>(int64_t, int, float*, double, const float*, double,
@@ -1021,22 +1148,25 @@ void launch_sum_partials(int nblk, int B, const double* partial, double* out, hi
 int gram_partial_blocks(int64_t N, int B) {
   (void)B;
   int64_t g = (N + kGramChunk - 1) / kGramChunk;
-  if (g > 1024) g = 1024;
+  if (g > 512) g = 512;
   return (int)(g < 1 ? 1 : g);
 }
 
 void launch_gram(int64_t N, int B, const double* X, const double* Y, double* partial, double* G,
                  hipStream_t s) {
-  if (B * B > 32 * kThreads) throw Error(IDFT_EINVAL, "block too wide for Gram kernel");
+  if (B > kMaxCols) throw Error(IDFT_EINVAL, "block too wide for Gram kernel");
   const int nb = gram_partial_blocks(N, B);
-  k_gram<<<nb, kThreads, 2 * kGramChunk * B * sizeof(double), s>>>(N, B, X, Y, partial);
+  dim3 grid(nb, (B * B + kThreads - 1) / kThreads);
+  k_gram<<<grid, kThreads, 2 * kGramChunk * B * sizeof(double), s>>>(N, B, X, Y, partial);
   IDFT_HIP(hipGetLastError());
   launch_sum_partials(nb, B * B, partial, G, s);
 }
 
 void launch_right_mult(int64_t N, int B, const double* X, const double* Q, double* Y,
                        const double* ns, double beta, hipStream_t s) {
-  k_right_mult<<<grid_for(N * B), kThreads, B * B * sizeof(double), s>>>(N, B, X, Q, Y, ns, beta);
+  check_u32(N * B);
+  k_right_mult<<<grid_for(N * B), kThreads, B * B * sizeof(double), s>>>((uint32_t)(N * B),
+                                                                         (uint32_t)B, X, Q, Y, ns, beta);
   IDFT_HIP(hipGetLastError());
 }
 
@@ -1140,13 +1270,15 @@ void launch_force(const Geo& g, const DevTables& t, const double* Psi, const dou
 }
 
 void launch_zero_rows(int64_t N, int B, double* X, const unsigned char* mask, hipStream_t s) {
-  k_zero_rows<<<grid_for(N * B), kThreads, 0, s>>>(N, B, X, mask);
+  check_u32(N * B);
+  k_zero_rows<<<grid_for(N * B), kThreads, 0, s>>>((uint32_t)(N * B), (uint32_t)B, X, mask);
   IDFT_HIP(hipGetLastError());
 }
 
 
-void launch_minres_start(int B, int nblk, const double* partial, double* st, hipStream_t s) {
-  k_minres_start<<<(B + 63) / 64, 64, 0, s>>>(B, nblk, partial, st);
+void launch_minres_start(int B, int nblk, const double* partial, const double* negEig, double* st,
+                         hipStream_t s) {
+  k_minres_start<<<(B + 63) / 64, 64, 0, s>>>(B, nblk, partial, negEig, st);
   IDFT_HIP(hipGetLastError());
 }
 void launch_minres_alpha(int B, int nblk, const double* partial, double* st, hipStream_t s) {
@@ -1165,16 +1297,31 @@ template int launch_minres_lanczos<float>(int64_t, int, float*, const float*, fl
                                           const double*, const double*, double*, hipStream_t);
 template int launch_minres_lanczos<double>(int64_t, int, double*, const double*, double*,
                                            const double*, const double*, double*, hipStream_t);
-void launch_minres_givens(int B, int nblk, const double* partial, double* st, hipStream_t s) {
-  k_minres_givens<<<(B + 63) / 64, 64, 0, s>>>(B, nblk, partial, st);
+void launch_minres_givens(int B, int nblk, const double* partial, const double* negEig, double* st,
+                          hipStream_t s) {
+  k_minres_givens<<<(B + 63) / 64, 64, 0, s>>>(B, nblk, partial, negEig, st);
   IDFT_HIP(hipGetLastError());
 }
 template <typename T>
 void launch_minres_update(int64_t N, int B, const T* v, T* w1, const T* w, double* x,
                           const double* st, hipStream_t s) {
-  k_minres_update<T><<<grid_for(N * B), kThreads, 0, s>>>(N, B, v, w1, w, x, st);
+  check_u32(N * B);
+  k_minres_update<T><<<grid_for(N * B), kThreads, 0, s>>>((uint32_t)(N * B), (uint32_t)B, v, w1, w,
+                                                          x, st);
   IDFT_HIP(hipGetLastError());
 }
+template <typename T>
+void launch_minres_prep(int64_t N, int B, T* y, T* v, const T* r1, const double* negEig,
+                        const double* st, hipStream_t s) {
+  check_u32(N * B);
+  k_minres_prep<T><<<grid_for(N * B), kThreads, 0, s>>>((uint32_t)(N * B), (uint32_t)B, y, v, r1,
+                                                        negEig, st);
+  IDFT_HIP(hipGetLastError());
+}
+template void launch_minres_prep<float>(int64_t, int, float*, float*, const float*, const double*,
+                                        const double*, hipStream_t);
+template void launch_minres_prep<double>(int64_t, int, double*, double*, const double*,
+                                         const double*, const double*, hipStream_t);
 template void launch_minres_update<float>(int64_t, int, const float*, float*, const float*,
                                           double*, const double*, hipStream_t);
 template void launch_minres_update<double>(int64_t, int, const double*, double*, const double*,
