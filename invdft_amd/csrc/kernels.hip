@@ -17,6 +17,22 @@ namespace {
 
 constexpr int kThreads = 256;
 
+// Sum of partial[b * B + j] over b < nblk by one 256-thread block (fixed
+// order: strided per-thread sums, then an LDS tree), result on thread 0.
+__device__ inline double block_sum_col(const double* __restrict__ partial, int nblk, int B, int j,
+                                       double* red) {
+  const int tid = threadIdx.x;
+  double s = 0.0;
+  for (int b = tid; b < nblk; b += kThreads) s += partial[(int64_t)b * B + j];
+  red[tid] = s;
+  __syncthreads();
+  for (int st = kThreads / 2; st > 0; st >>= 1) {
+    if (tid < st) red[tid] += red[tid + st];
+    __syncthreads();
+  }
+  return red[0];
+}
+
 inline void check_u32(int64_t n) {
   if (n >= (int64_t)0xffffffffLL) throw Error(IDFT_EINVAL, "multivector too large for 32-bit indexing");
 }
@@ -680,13 +696,13 @@ __global__ __launch_bounds__(kThreads) void k_coldot(int64_t N, int B, const T* 
   }
 }
 
-__global__ void k_sum_partials(int nblk, int B, const double* __restrict__ partial,
-                               double* __restrict__ out) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= B) return;
-  double s = 0.0;
-  for (int b = 0; b < nblk; ++b) s += partial[(int64_t)b * B + j];
-  out[j] = s;
+__global__ __launch_bounds__(kThreads) void k_sum_partials(int nblk, int B,
+                                                            const double* __restrict__ partial,
+                                                            double* __restrict__ out) {
+  __shared__ double red[kThreads];
+  const int j = blockIdx.x;
+  const double s = block_sum_col(partial, nblk, B, j, red);
+  if (threadIdx.x == 0) out[j] = s;
 }
 
 // Gram partials: block (bx, by) accumulates the entries ij in
@@ -798,12 +814,14 @@ __global__ void k_zero_rows(uint32_t NB, uint32_t B, double* __restrict__ X,
 
 // ------------------------------------------------------------- MINRES
 // Per-column state st[k * B + j] (see MinresSlot); one thread per column.
-__global__ void k_minres_start(int B, int nblk, const double* __restrict__ partial,
-                               const double* __restrict__ negEig, double* __restrict__ st) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= B) return;
-  double s = 0.0;
-  for (int b = 0; b < nblk; ++b) s += partial[(int64_t)b * B + j];
+__global__ __launch_bounds__(kThreads) void k_minres_start(int B, int nblk,
+                                                            const double* __restrict__ partial,
+                                                            const double* __restrict__ negEig,
+                                                            double* __restrict__ st) {
+  __shared__ double red[kThreads];
+  const int j = blockIdx.x;
+  const double s = block_sum_col(partial, nblk, B, j, red);
+  if (threadIdx.x != 0) return;
   const double beta = sqrt(fmax(s, 0.0));
   st[MS_BETA * B + j] = beta;
   st[MS_OLDB * B + j] = 0.0;
@@ -821,12 +839,13 @@ __global__ void k_minres_start(int B, int nblk, const double* __restrict__ parti
   st[MS_CR1 * B + j] = 0.0;
 }
 
-__global__ void k_minres_alpha(int B, int nblk, const double* __restrict__ partial,
-                               double* __restrict__ st) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= B) return;
-  double s = 0.0;
-  for (int b = 0; b < nblk; ++b) s += partial[(int64_t)b * B + j];
+__global__ __launch_bounds__(kThreads) void k_minres_alpha(int B, int nblk,
+                                                            const double* __restrict__ partial,
+                                                            double* __restrict__ st) {
+  __shared__ double red[kThreads];
+  const int j = blockIdx.x;
+  const double s = block_sum_col(partial, nblk, B, j, red);
+  if (threadIdx.x != 0) return;
   st[MS_ALFA * B + j] = s;
   const bool done = st[MS_DONE * B + j] != 0.0;
   st[MS_CA2 * B + j] = done ? 0.0 : -s / st[MS_BETA * B + j];
@@ -866,10 +885,14 @@ __global__ __launch_bounds__(kThreads) void k_minres_lanczos(int64_t N, int B, T
   }
 }
 
-__global__ void k_minres_givens(int B, int nblk, const double* __restrict__ partial,
-                                const double* __restrict__ negEig, double* __restrict__ st) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= B) return;
+__global__ __launch_bounds__(kThreads) void k_minres_givens(int B, int nblk,
+                                                             const double* __restrict__ partial,
+                                                             const double* __restrict__ negEig,
+                                                             double* __restrict__ st) {
+  __shared__ double red[kThreads];
+  const int j = blockIdx.x;
+  const double s = block_sum_col(partial, nblk, B, j, red);
+  if (threadIdx.x != 0) return;
   double* S = st;
   if (S[MS_DONE * B + j] != 0.0) {
     S[MS_CWV * B + j] = 0.0; S[MS_CWW1 * B + j] = 0.0; S[MS_CWW * B + j] = 0.0;
@@ -877,8 +900,6 @@ __global__ void k_minres_givens(int B, int nblk, const double* __restrict__ part
     S[MS_CVN * B + j] = 0.0;
     return;
   }
-  double s = 0.0;
-  for (int b = 0; b < nblk; ++b) s += partial[(int64_t)b * B + j];
   const double oldb = S[MS_BETA * B + j];
   const double beta = sqrt(fmax(s, 0.0));
   const double alfa = S[MS_ALFA * B + j];
@@ -1141,7 +1162,7 @@ template int launch_coldot_partial<double>(int64_t, int, const double*, const do
                                            const double*, double*, hipStream_t);
 
 void launch_sum_partials(int nblk, int B, const double* partial, double* out, hipStream_t s) {
-  k_sum_partials<<<(B + 63) / 64, 64, 0, s>>>(nblk, B, partial, out);
+  k_sum_partials<<<B, kThreads, 0, s>>>(nblk, B, partial, out);
   IDFT_HIP(hipGetLastError());
 }
 
@@ -1278,11 +1299,11 @@ void launch_zero_rows(int64_t N, int B, double* X, const unsigned char* mask, hi
 
 void launch_minres_start(int B, int nblk, const double* partial, const double* negEig, double* st,
                          hipStream_t s) {
-  k_minres_start<<<(B + 63) / 64, 64, 0, s>>>(B, nblk, partial, negEig, st);
+  k_minres_start<<<B, kThreads, 0, s>>>(B, nblk, partial, negEig, st);
   IDFT_HIP(hipGetLastError());
 }
 void launch_minres_alpha(int B, int nblk, const double* partial, double* st, hipStream_t s) {
-  k_minres_alpha<<<(B + 63) / 64, 64, 0, s>>>(B, nblk, partial, st);
+  k_minres_alpha<<<B, kThreads, 0, s>>>(B, nblk, partial, st);
   IDFT_HIP(hipGetLastError());
 }
 template <typename T>
@@ -1299,7 +1320,7 @@ template int launch_minres_lanczos<double>(int64_t, int, double*, const double*,
                                            const double*, const double*, double*, hipStream_t);
 void launch_minres_givens(int B, int nblk, const double* partial, const double* negEig, double* st,
                           hipStream_t s) {
-  k_minres_givens<<<(B + 63) / 64, 64, 0, s>>>(B, nblk, partial, negEig, st);
+  k_minres_givens<<<B, kThreads, 0, s>>>(B, nblk, partial, negEig, st);
   IDFT_HIP(hipGetLastError());
 }
 template <typename T>
