@@ -38,6 +38,11 @@ def log(*a):
 def run_bfgs(lib, prob, precision, warmup, steps, tol_kw):
     from invdft_amd import abi
     ctx = prob.context(lib, precision=precision, **tol_kw)
+    ctx.set_potential(prob.vxc0)  # cell matrices for the autotuner (untimed)
+    tuned = {"fp64": ctx.autotune_apply(prob.desc.n_eig, abi.FP64)}
+    if precision == abi.MIXED:
+        tuned["fp32"] = ctx.autotune_apply(prob.desc.n_eig, abi.MIXED)
+    log(f"  autotuned cell-product tile shapes: {tuned}")
     t0 = time.perf_counter()
     if warmup > 0:
         ctx.bfgs_solve(ls_type=abi.LS_CP, max_iter=warmup)
@@ -56,7 +61,7 @@ def run_bfgs(lib, prob, precision, warmup, steps, tol_kw):
         f"refinements={st1['refinements']} t_eig={st1['t_eigen_ms']:.1f} "
         f"t_adj={st1['t_adjoint_ms']:.1f} t_force={st1['t_force_ms']:.1f} ms")
     return ctx, dict(iters=n, timed_ms=timed_ms, s_per_iter=timed_ms / 1000.0 / n, loss=list(loss),
-                     gnorm=list(gnorm), per_eval=per_eval, last=st1, evals=evals)
+                     gnorm=list(gnorm), per_eval=per_eval, last=st1, evals=evals, tuned=tuned)
 
 
 def cpu_baseline(prob, per_eval, evals_per_iter):
@@ -189,6 +194,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "detail": {"mixed_per_eval": mixed["per_eval"], "fp64_per_eval": fp64["per_eval"] if fp64 else None,
+                   "tile_shapes": {"mixed": mixed["tuned"], "fp64": fp64["tuned"] if fp64 else None},
                    "mixed_loss": mixed["loss"], **extra},
     }
     print(json.dumps(out), flush=True)

@@ -135,18 +135,23 @@ struct ApplyEpilogue {
   double* dot_partial = nullptr;
 };
 constexpr int kGatherBlocks = 2048;
-int gather_blocks(int64_t N, int B);
+int gather_blocks(const Geo& g, int B);  // rows of the gather's dot partials
 // Cell operator in two launches: per-cell H_e (S xcol Xin)_e into the scratch
 // Yc (cell_scratch_elems(g, B) elements), then the node gather + epilogue.
 // Returns the number of gather blocks (rows of dot_partial).
+// Tile shape of the MFMA cell-product kernel (chosen by the autotuner).
+struct GemmShape {
+  int cells_per_wg = 1;       // cells processed in sequence by one work-group
+  int vectors_per_tile = 80;  // columns per launch tile (rounded to 16; >= B: one pass)
+};
 template <typename T>
 int launch_cell_apply(const Geo& g, const T* H, const T* S, const T* Xin, T* Yc, int B,
-                      const ApplyEpilogue<T>& ep, hipStream_t s);
+                      const ApplyEpilogue<T>& ep, const GemmShape& shape, hipStream_t s);
 int64_t cell_scratch_elems(const Geo& g, int B);
 // Phase 1 alone (per-cell products into Yc), for profiling.
 template <typename T>
 void launch_cell_gemm(const Geo& g, const T* H, const T* S, const T* X, const double* xcol, T* Yc,
-                      int B, hipStream_t s);
+                      int B, const GemmShape& shape, hipStream_t s);
 // Y = beta Y + ns[n] (a ca[j] X + b cb[j] Z); null pointers read as 1 (X/Z as 0).
 template <typename T>
 void launch_lincomb(int64_t N, int B, T* Y, double beta, const T* X, double a, const double* ca,
@@ -213,7 +218,7 @@ int launch_minres_lanczos(int64_t N, int B, T* y, const T* r2, T* rnew, const do
 void launch_minres_givens(int B, int nblk, const double* partial, const double* negEig,
                           double* st, hipStream_t s);
 template <typename T>
-void launch_minres_update(int64_t N, int B, const T* v, T* w1, const T* w, double* x,
+void launch_minres_update(int64_t N, int B, const T* v, T* w1, const T* w, T* x,
                           const double* st, hipStream_t s);
 void launch_copy_done(int B, const double* st, int* flags, hipStream_t s);
 template <typename T>

@@ -70,10 +70,11 @@ struct idft_ctx {
   // Hamiltonian
   DevBuf<double> H64, Yc64;
   DevBuf<float> H32, Yc32;
+  GemmShape shape64, shape32;  // cell-product tile shapes (autotuned)
   bool haveH = false;
   // eigen state (Loewdin vectors) and work
   DevBuf<double> X, X2, HX, Xtmp, T2buf, rref, psiAll, force;
-  DevBuf<float> F0, F1, F2, Rf;
+  DevBuf<float> F0, F1, F2, Rf, x32;
   DevBuf<double> ydev;
   bool haveHX = false;  // HX == H~ X for the current X and Hamiltonian
   DevBuf<double> partial, small, small2, occD;
@@ -106,13 +107,13 @@ struct idft_ctx {
       Yc64.resize(cell_scratch_elems(g, b));
       st.applies_fp64++;
       st.apply_cols_fp64 += b;
-      return launch_cell_apply<double>(g, H64.p, S.p, Xin, Yc64.p, b, ep, s);
+      return launch_cell_apply<double>(g, H64.p, S.p, Xin, Yc64.p, b, ep, shape64, s);
     } else {
       if (!H32.p) throw Error(IDFT_ESTATE, "FP32 Hamiltonian not built (precision is FP64)");
       Yc32.resize(cell_scratch_elems(g, b));
       st.applies_fp32++;
       st.apply_cols_fp32 += b;
-      return launch_cell_apply<float>(g, H32.p, S32.p, Xin, Yc32.p, b, ep, s);
+      return launch_cell_apply<float>(g, H32.p, S32.p, Xin, Yc32.p, b, ep, shape32, s);
     }
   }
   // Y = alpha S H S X + beta Y
@@ -617,8 +618,9 @@ struct idft_ctx {
   }
 
   // Preconditioned MINRES from x = 0 (x accumulates), per-column tolerance.
+  // x (same precision as the recurrence) accumulates the solution increment.
   template <typename T>
-  std::vector<int> minres_core(const T* rhs, double* x, int b, const std::vector<double>& tol,
+  std::vector<int> minres_core(const T* rhs, T* x, int b, const std::vector<double>& tol,
                                int maxit) {
     MinresWork<T>& W = work<T>();
     W.ensure(NB(b));
@@ -633,7 +635,7 @@ struct idft_ctx {
     launch_lincomb<T>(N, b, W.y.p, 0.0, rhs, 1.0, nullptr, nullptr, 0.0, nullptr, Djac.p, s);
     launch_lincomb<T>(N, b, W.w1.p, 0.0, nullptr, 0.0, nullptr, nullptr, 0.0, nullptr, nullptr, s);
     launch_lincomb<T>(N, b, W.w.p, 0.0, nullptr, 0.0, nullptr, nullptr, 0.0, nullptr, nullptr, s);
-    partial.resize((size_t)std::max(1024, kGatherBlocks) * b);
+    partial.resize((size_t)std::max(1024, gather_blocks(g, b)) * b);
     int nb = launch_coldot_partial<T>(N, b, W.r1.p, W.y.p, nullptr, partial.p, s);
     launch_minres_start(b, nb, partial.p, adj.dnegEig.p, mst.p, s);
     const int check = std::max(1, d.check_every);
@@ -717,7 +719,12 @@ struct idft_ctx {
         }
         if (all) break;
         launch_convert_d2f((int64_t)NB(b), mw32.rhs.p, r, s);
-        std::vector<int> it = minres_core<float>(mw32.rhs.p, xp, b, tl, maxit - used);
+        x32.resize(NB(b));
+        launch_lincomb<float>(t.N, b, x32.p, 0.0, nullptr, 0.0, nullptr, nullptr, 0.0, nullptr, nullptr, s);
+        std::vector<int> it = minres_core<float>(mw32.rhs.p, x32.p, b, tl, maxit - used);
+        // x += delta (FP32 increment, FP64 accumulation)
+        launch_convert_f2d((int64_t)NB(b), r, x32.p, s);
+        launch_lincomb<double>(t.N, b, xp, 1.0, r, 1.0, nullptr, nullptr, 0.0, nullptr, nullptr, s);
         int mx = 0;
         for (int j = 0; j < b; ++j) { total[j] += it[j]; mx = std::max(mx, it[j]); }
         used += std::max(mx, 1);
@@ -1229,8 +1236,8 @@ int idft_profile_apply(idft_ctx* c, int32_t ncols, int32_t prec, int32_t reps, d
       else c->happly64(xd.p, yd.p, ncols, 1.0);
     };
     auto gemm = [&] {
-      if (f32) launch_cell_gemm<float>(c->g, c->H32.p, c->S32.p, xf.p, nullptr, c->Yc32.p, ncols, c->s);
-      else launch_cell_gemm<double>(c->g, c->H64.p, c->S.p, xd.p, nullptr, c->Yc64.p, ncols, c->s);
+      if (f32) launch_cell_gemm<float>(c->g, c->H32.p, c->S32.p, xf.p, nullptr, c->Yc32.p, ncols, c->shape32, c->s);
+      else launch_cell_gemm<double>(c->g, c->H64.p, c->S.p, xd.p, nullptr, c->Yc64.p, ncols, c->shape64, c->s);
     };
     hipEvent_t e0, e1;
     IDFT_HIP(hipEventCreate(&e0));
@@ -1250,6 +1257,51 @@ int idft_profile_apply(idft_ctx* c, int32_t ncols, int32_t prec, int32_t reps, d
     if (ms_gemm) *ms_gemm = timed(gemm);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+  });
+}
+
+int idft_set_apply_shape(idft_ctx* c, int32_t prec, int32_t cells_per_wg,
+                         int32_t vectors_per_tile) {
+  if (!c || cells_per_wg < 1 || cells_per_wg > 64 || vectors_per_tile < 16 ||
+      vectors_per_tile > kMaxCols || vectors_per_tile % 16)
+    return IDFT_EINVAL;
+  GemmShape& t = prec == IDFT_MIXED ? c->shape32 : c->shape64;
+  t.cells_per_wg = cells_per_wg;
+  t.vectors_per_tile = vectors_per_tile;
+  return IDFT_OK;
+}
+
+int idft_autotune_apply(idft_ctx* c, int32_t ncols, int32_t prec, int32_t reps,
+                        int32_t* cells_per_wg, int32_t* vectors_per_tile, double* ms_best) {
+  if (!c || ncols < 1 || ncols > kMaxCols || reps < 1) return IDFT_EINVAL;
+  if (!c->haveH) { c->err = "no Hamiltonian"; return IDFT_ESTATE; }
+  const bool f32 = prec == IDFT_MIXED;
+  if (f32 && !c->H32.p) { c->err = "FP32 Hamiltonian not built"; return IDFT_ESTATE; }
+  return guarded(c, [&] {
+    GemmShape& target = f32 ? c->shape32 : c->shape64;
+    const GemmShape saved = target;
+    GemmShape best = saved;
+    double bestMs = 1e300;
+    const int vmax = ((ncols + 15) / 16) * 16;
+    std::vector<int> widths;
+    for (int vt : {16, 32, 64, vmax}) {
+      const int e = std::min(vt, vmax);
+      if (std::find(widths.begin(), widths.end(), e) == widths.end()) widths.push_back(e);
+    }
+    for (int vt : widths) {
+      for (int cp : {1, 2, 4, 8}) {
+        target.cells_per_wg = cp;
+        target.vectors_per_tile = vt;
+        double ms = 0.0;
+        int rc = idft_profile_apply(c, ncols, prec, reps, &ms, nullptr);
+        if (rc) { target = saved; throw Error(rc, c->err); }
+        if (ms < bestMs) { bestMs = ms; best = target; }
+      }
+    }
+    target = best;
+    if (cells_per_wg) *cells_per_wg = best.cells_per_wg;
+    if (vectors_per_tile) *vectors_per_tile = best.vectors_per_tile;
+    if (ms_best) *ms_best = bestMs;
   });
 }
 

@@ -9,6 +9,8 @@
 //                 H[rt*16 + l%16][kg*16 + s*4 + l/16],  s = 0..3.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "common.h"
 
 namespace idft {
@@ -99,81 +101,88 @@ template <> struct Vec4<double> { typedef double4 type; };
 
 __host__ __device__ constexpr int ldb_of(int bt) { return (bt % 32 == 16) ? bt : bt + 16; }
 
-// Phase 1 of the cell operator.  One work-group = one cell; the block (S X)_e
-// is gathered into LDS, the 4 waves share the 16-row tiles of H_e and stream
-// their tiles from HBM with one 16- (f32) or 32-byte (f64) load per lane per
-// four MFMA k-steps, prefetching KPF k-groups ahead.  Output: Yc[e][l][j] =
-// (H_e (S X)_e)[l][j] for j < B.  Work-groups are dealt to the 8 XCDs in
-// contiguous runs of cells.
+// Phase 1 of the cell operator.  One work-group handles `cpw` consecutive
+// cells and the column tile [blockIdx.y * 16 NCT, +16 NCT) of the block.  Per
+// cell the tile of (S X)_e is gathered into LDS, the 4 waves share the 16-row
+// tiles of H_e and stream them from HBM with one 16- (f32) or 32-byte (f64)
+// load per lane per four MFMA k-steps, KPF k-groups in flight per wave.
+// Output: Yc[e][l][j] = (H_e (S X)_e)[l][j].  Work-groups are dealt to the 8
+// XCDs in contiguous runs of cells.
 template <typename T, int P, int NCT>
 __global__ __launch_bounds__(kThreads) void k_cell_gemm(Geo g, const T* __restrict__ H,
                                                         const T* __restrict__ S,
                                                         const T* __restrict__ X,
                                                         const double* __restrict__ xcol,
-                                                        T* __restrict__ Yc, int B) {
+                                                        T* __restrict__ Yc, int B, int cpw) {
   constexpr int NE = ne_of(P), NEP = nep_of(P), RT = NEP / 16, BT = NCT * 16;
   constexpr int LDB = ldb_of(BT);
-  constexpr int KPF = 3;  // k-groups in flight per wave
+  constexpr int KPF = 3;
   __shared__ T xs[NEP * LDB];
-  const int64_t nw = g.ncell;
-  const int64_t per = (nw + 7) / 8;
-  const int64_t e = (blockIdx.x % 8) * per + blockIdx.x / 8;
-  if (e >= nw) return;
-  const int64_t cx = e % g.nc, cy = (e / g.nc) % g.nc, cz = e / ((int64_t)g.nc * g.nc);
+  const int64_t nwg = (g.ncell + cpw - 1) / cpw;
+  const int64_t per = (nwg + 7) / 8;
+  const int64_t wg = (blockIdx.x % 8) * per + blockIdx.x / 8;
+  if (wg >= nwg) return;
+  const int c0 = blockIdx.y * BT;
+  const int bw = min(BT, B - c0);  // columns of this tile
   const int tid = threadIdx.x;
-  for (int i = tid; i < NEP * BT; i += kThreads) {
-    const int l = i / BT, j = i % BT;
-    T v = T(0);
-    if (l < NE && j < B) {
-      const int64_t n = cell_node(g, cx, cy, cz, l);
-      v = S[n] * X[n * B + j];
-      if (xcol) v *= (T)xcol[j];
-    }
-    xs[l * LDB + j] = v;
-  }
-  __syncthreads();
   const int wave = tid / 64, lane = tid % 64;
   const int lr = lane % 16, lk = lane / 16;
   typedef typename Mfma<T>::acc_t acc_t;
   typedef typename Vec4<T>::type vec_t;
-  const vec_t* Ht = reinterpret_cast<const vec_t*>(H + (size_t)e * NEP * NEP);
-  T* yc = Yc + (size_t)e * NE * B;
-  for (int rt = wave; rt < RT; rt += 4) {
-    acc_t acc[NCT];
+  const int64_t e0 = wg * cpw, e1 = min((int64_t)g.ncell, e0 + cpw);
+  for (int64_t e = e0; e < e1; ++e) {
+    const int64_t cx = e % g.nc, cy = (e / g.nc) % g.nc, cz = e / ((int64_t)g.nc * g.nc);
+    if (e > e0) __syncthreads();  // previous cell's tiles fully consumed
+    for (int i = tid; i < NEP * BT; i += kThreads) {
+      const int l = i / BT, j = i % BT;
+      T v = T(0);
+      if (l < NE && j < bw) {
+        const int64_t n = cell_node(g, cx, cy, cz, l);
+        v = S[n] * X[n * B + c0 + j];
+        if (xcol) v *= (T)xcol[c0 + j];
+      }
+      xs[l * LDB + j] = v;
+    }
+    __syncthreads();
+    const vec_t* Ht = reinterpret_cast<const vec_t*>(H + (size_t)e * NEP * NEP);
+    T* yc = Yc + (size_t)e * NE * B + c0;
+    for (int rt = wave; rt < RT; rt += 4) {
+      acc_t acc[NCT];
 #pragma unroll
-    for (int ct = 0; ct < NCT; ++ct) acc[ct] = acc_t{0, 0, 0, 0};
-    const vec_t* row = Ht + (size_t)rt * RT * 64 + lane;
-    vec_t buf[KPF];
+      for (int ct = 0; ct < NCT; ++ct) acc[ct] = acc_t{0, 0, 0, 0};
+      const vec_t* row = Ht + (size_t)rt * RT * 64 + lane;
+      vec_t buf[KPF];
 #pragma unroll
-    for (int k = 0; k < KPF; ++k) buf[k] = k < RT ? row[k * 64] : vec_t{};
+      for (int kk = 0; kk < KPF; ++kk) buf[kk] = kk < RT ? row[kk * 64] : vec_t{};
 #pragma unroll 1
-    for (int kg0 = 0; kg0 < RT; kg0 += KPF) {
+      for (int kg0 = 0; kg0 < RT; kg0 += KPF) {
 #pragma unroll
-      for (int k = 0; k < KPF; ++k) {
-        const int kg = kg0 + k;
-        if (kg < RT) {
-          const vec_t a = buf[k];
-          if (kg + KPF < RT) buf[k] = row[(kg + KPF) * 64];
-          const T* xk = xs + (kg * 16 + lk) * LDB + lr;
-          const T av[4] = {a.x, a.y, a.z, a.w};
+        for (int kk = 0; kk < KPF; ++kk) {
+          const int kg = kg0 + kk;
+          if (kg < RT) {
+            const vec_t a = buf[kk];
+            if (kg + KPF < RT) buf[kk] = row[(kg + KPF) * 64];
+            const T* xk = xs + (kg * 16 + lk) * LDB + lr;
+            const T av[4] = {a.x, a.y, a.z, a.w};
 #pragma unroll
-          for (int s = 0; s < 4; ++s) {
+            for (int s = 0; s < 4; ++s) {
 #pragma unroll
-            for (int ct = 0; ct < NCT; ++ct)
-              acc[ct] = Mfma<T>::run(av[s], xk[s * 4 * LDB + ct * 16], acc[ct]);
+              for (int ct = 0; ct < NCT; ++ct)
+                acc[ct] = Mfma<T>::run(av[s], xk[s * 4 * LDB + ct * 16], acc[ct]);
+            }
           }
         }
       }
-    }
-    // lane holds rows rt*16 + Mfma<T>::row(lk, i), column ct*16 + lr
+      // lane holds rows rt*16 + Mfma<T>::row(lk, i), column ct*16 + lr
 #pragma unroll
-    for (int ct = 0; ct < NCT; ++ct) {
-      const int col = ct * 16 + lr;
-      if (col >= B) continue;
+      for (int ct = 0; ct < NCT; ++ct) {
+        const int col = ct * 16 + lr;
+        if (col >= bw) continue;
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int r = rt * 16 + Mfma<T>::row(lk, i);
-        if (r < NE) yc[(size_t)r * B + col] = acc[ct][i];
+        for (int i = 0; i < 4; ++i) {
+          const int r = rt * 16 + Mfma<T>::row(lk, i);
+          if (r < NE) yc[(size_t)r * B + col] = acc[ct][i];
+        }
       }
     }
   }
@@ -184,45 +193,58 @@ __global__ __launch_bounds__(kThreads) void k_cell_gemm(Geo g, const T* __restri
 // no colouring -- and combines them with up to three other blocks:
 //   Y = beta Y + alpha S[n] sum + a ca[j] X + b cb[j] Z + c cc[j] W.
 // Optionally V = xcol[j] Xin (the scaled operator input) and per-column block
-// partials of V.Y (FP64).  Thread t owns column t % B (as in k_coldot).
-template <typename T>
+// partials of V.Y (FP64).  Block (bx, by) covers node rows (iy, iz) in
+// [by * R, by * R + R) and x positions [bx * per, bx * per + per) of each row;
+// thread t owns column t % B and x offset t / B for its whole life.
+template <typename T, int P>
 __global__ __launch_bounds__(kThreads) void k_cell_gather(
-    Geo g, uint32_t B, const T* __restrict__ Yc, const T* __restrict__ S, T* __restrict__ Y,
-    double alpha, double beta, const T* __restrict__ X, double a, const double* __restrict__ ca,
-    const T* __restrict__ Z, double b, const double* __restrict__ cb, const T* __restrict__ W,
-    double c, const double* __restrict__ cc, const T* __restrict__ Xin,
-    const double* __restrict__ xcol, T* __restrict__ V, double* __restrict__ partial) {
+    Geo g, uint32_t B, uint32_t R, const T* __restrict__ Yc, const T* __restrict__ S,
+    T* __restrict__ Y, double alpha, double beta, const T* __restrict__ X, double a,
+    const double* __restrict__ ca, const T* __restrict__ Z, double b,
+    const double* __restrict__ cb, const T* __restrict__ W, double c,
+    const double* __restrict__ cc, const T* __restrict__ Xin, const double* __restrict__ xcol,
+    T* __restrict__ V, double* __restrict__ partial) {
+  constexpr uint32_t n1 = P + 1, ne = n1 * n1 * n1;
   __shared__ double red[kThreads];
-  const uint32_t N1 = g.N1, p = g.p, n1 = g.n1, nc = g.nc;
-  const uint32_t ne = n1 * n1 * n1;
-  const uint32_t N = (uint32_t)g.N;
+  const uint32_t N1 = g.N1, nc = g.nc;
   const uint32_t tid = threadIdx.x, per = kThreads / B;
   const uint32_t j = tid % B, off = tid / B;
+  const uint32_t ix = blockIdx.x * per + off;
   double dot = 0.0;
-  if (off < per) {
+  if (off < per && ix < N1) {
     const double caj = X ? a * (ca ? ca[j] : 1.0) : 0.0;
     const double cbj = Z ? b * (cb ? cb[j] : 1.0) : 0.0;
     const double ccj = W ? c * (cc ? cc[j] : 1.0) : 0.0;
     const double xs = xcol ? xcol[j] : 1.0;
-    for (uint32_t n = blockIdx.x * per + off; n < N; n += gridDim.x * per) {
-      const uint32_t i = n * B + j;
-      const uint32_t ix = n % N1, iy = (n / N1) % N1, iz = n / (N1 * N1);
-      uint32_t cxs[2], axs[2], cys[2], ays[2], czs[2], azs[2];
-      int nx = 0, ny = 0, nz = 0;
-      const uint32_t qx = ix / p, rx = ix - qx * p, qy = iy / p, ry = iy - qy * p,
-                     qz = iz / p, rz = iz - qz * p;
-      if (rx == 0) {
-        if (qx > 0) { cxs[nx] = qx - 1; axs[nx++] = p; }
-        if (qx < nc) { cxs[nx] = qx; axs[nx++] = 0; }
-      } else { cxs[nx] = qx; axs[nx++] = rx; }
-      if (ry == 0) {
-        if (qy > 0) { cys[ny] = qy - 1; ays[ny++] = p; }
-        if (qy < nc) { cys[ny] = qy; ays[ny++] = 0; }
-      } else { cys[ny] = qy; ays[ny++] = ry; }
-      if (rz == 0) {
-        if (qz > 0) { czs[nz] = qz - 1; azs[nz++] = p; }
-        if (qz < nc) { czs[nz] = qz; azs[nz++] = 0; }
-      } else { czs[nz] = qz; azs[nz++] = rz; }
+    // x contributions (cell, local index) of this thread's column of nodes
+    uint32_t cxs[2], axs[2];
+    int nx = 0;
+    {
+      const uint32_t q = ix / P, r = ix - q * P;
+      if (r == 0) {
+        if (q > 0) { cxs[nx] = q - 1; axs[nx++] = P; }
+        if (q < nc) { cxs[nx] = q; axs[nx++] = 0; }
+      } else { cxs[nx] = q; axs[nx++] = r; }
+    }
+    const uint32_t row0 = blockIdx.y * R, row1 = min(row0 + R, N1 * N1);
+    for (uint32_t row = row0; row < row1; ++row) {
+      const uint32_t iy = row % N1, iz = row / N1;
+      uint32_t cys[2], ays[2], czs[2], azs[2];
+      int ny = 0, nz = 0;
+      {
+        const uint32_t q = iy / P, r = iy - q * P;
+        if (r == 0) {
+          if (q > 0) { cys[ny] = q - 1; ays[ny++] = P; }
+          if (q < nc) { cys[ny] = q; ays[ny++] = 0; }
+        } else { cys[ny] = q; ays[ny++] = r; }
+      }
+      {
+        const uint32_t q = iz / P, r = iz - q * P;
+        if (r == 0) {
+          if (q > 0) { czs[nz] = q - 1; azs[nz++] = P; }
+          if (q < nc) { czs[nz] = q; azs[nz++] = 0; }
+        } else { czs[nz] = q; azs[nz++] = r; }
+      }
       double sum = 0.0;
       for (int kz = 0; kz < nz; ++kz)
         for (int ky = 0; ky < ny; ++ky)
@@ -231,6 +253,8 @@ __global__ __launch_bounds__(kThreads) void k_cell_gather(
             const uint32_t l = (azs[kz] * n1 + ays[ky]) * n1 + axs[kx];
             sum += (double)Yc[(e * ne + l) * B + j];
           }
+      const uint32_t n = row * N1 + ix;
+      const uint32_t i = n * B + j;
       double v = alpha * (double)S[n] * sum;
       if (X) v += caj * (double)X[i];
       if (Z) v += cbj * (double)Z[i];
@@ -251,7 +275,7 @@ __global__ __launch_bounds__(kThreads) void k_cell_gather(
     if (tid < B) {
       double t = 0.0;
       for (uint32_t k = 0; k < per; ++k) t += red[k * B + tid];
-      partial[(int64_t)blockIdx.x * B + tid] = t;
+      partial[((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * B + tid] = t;
     }
   }
 }
@@ -936,13 +960,13 @@ __global__ __launch_bounds__(kThreads) void k_minres_givens(int B, int nblk,
 template <typename T>
 __global__ void k_minres_update(uint32_t NB, uint32_t B, const T* __restrict__ v,
                                 T* __restrict__ w1, const T* __restrict__ w,
-                                double* __restrict__ x, const double* __restrict__ st) {
+                                T* __restrict__ x, const double* __restrict__ st) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < NB; i += gridDim.x * blockDim.x) {
     const uint32_t j = i % B;
     const double wn = st[MS_CWV * B + j] * (double)v[i] + st[MS_CWW1 * B + j] * (double)w1[i] +
                       st[MS_CWW * B + j] * (double)w[i];
     w1[i] = (T)wn;
-    x[i] += st[MS_CX * B + j] * wn;
+    x[i] = (T)((double)x[i] + st[MS_CX * B + j] * wn);
   }
 }
 
@@ -1034,70 +1058,99 @@ void launch_build_hamiltonian(const Geo& g, const DevTables& t, double hh, const
 
 template <typename T, int P>
 static void cell_gemm_p(const Geo& g, const T* H, const T* S, const T* X, const double* xcol,
-                        T* Yc, int B, hipStream_t s) {
-  const int nct = (B + 15) / 16;
-  dim3 grid((unsigned)(8 * ((g.ncell + 7) / 8))), block(kThreads);
+                        T* Yc, int B, const GemmShape& shape, hipStream_t s) {
+  const int vt = std::min(shape.vectors_per_tile, ((B + 15) / 16) * 16);
+  const int nct = std::max(1, std::min(5, vt / 16));
+  const int cpw = std::max(1, shape.cells_per_wg);
+  const int64_t nwg = (g.ncell + cpw - 1) / cpw;
+  dim3 grid((unsigned)(8 * ((nwg + 7) / 8)), (unsigned)((B + 16 * nct - 1) / (16 * nct))),
+      block(kThreads);
   switch (nct) {
-    case 1: k_cell_gemm<T, P, 1><<<grid, block, 0, s>>>(g, H, S, X, xcol, Yc, B); break;
-    case 2: k_cell_gemm<T, P, 2><<<grid, block, 0, s>>>(g, H, S, X, xcol, Yc, B); break;
-    case 3: k_cell_gemm<T, P, 3><<<grid, block, 0, s>>>(g, H, S, X, xcol, Yc, B); break;
-    case 4: k_cell_gemm<T, P, 4><<<grid, block, 0, s>>>(g, H, S, X, xcol, Yc, B); break;
-    case 5: k_cell_gemm<T, P, 5><<<grid, block, 0, s>>>(g, H, S, X, xcol, Yc, B); break;
-    default: throw Error(IDFT_EINVAL, "block wider than 80 columns");
+    case 1: k_cell_gemm<T, P, 1><<<grid, block, 0, s>>>(g, H, S, X, xcol, Yc, B, cpw); break;
+    case 2: k_cell_gemm<T, P, 2><<<grid, block, 0, s>>>(g, H, S, X, xcol, Yc, B, cpw); break;
+    case 3: k_cell_gemm<T, P, 3><<<grid, block, 0, s>>>(g, H, S, X, xcol, Yc, B, cpw); break;
+    case 4: k_cell_gemm<T, P, 4><<<grid, block, 0, s>>>(g, H, S, X, xcol, Yc, B, cpw); break;
+    case 5: k_cell_gemm<T, P, 5><<<grid, block, 0, s>>>(g, H, S, X, xcol, Yc, B, cpw); break;
   }
 }
 
 int64_t cell_scratch_elems(const Geo& g, int B) { return g.ncell * (int64_t)ne_of(g.p) * B; }
 
-int gather_blocks(int64_t N, int B) {
+// Gather grid: x chunks of kThreads / B nodes times groups of R node rows,
+// with R chosen so that the grid has at most about kGatherBlocks blocks.
+static void gather_grid(const Geo& g, int B, unsigned& gx, unsigned& gy, uint32_t& R) {
   const int per = kThreads / B;
-  int64_t g = (N + per - 1) / per;
-  if (g > kGatherBlocks) g = kGatherBlocks;
-  return (int)(g < 1 ? 1 : g);
+  gx = (unsigned)((g.N1 + per - 1) / per);
+  const int64_t rows = (int64_t)g.N1 * g.N1;
+  int64_t r = (rows * gx + kGatherBlocks - 1) / kGatherBlocks;
+  if (r < 1) r = 1;
+  R = (uint32_t)r;
+  gy = (unsigned)((rows + r - 1) / r);
+}
+
+int gather_blocks(const Geo& g, int B) {
+  unsigned gx, gy;
+  uint32_t R;
+  gather_grid(g, B, gx, gy, R);
+  return (int)(gx * gy);
 }
 
 template <typename T>
 void launch_cell_gemm(const Geo& g, const T* H, const T* S, const T* X, const double* xcol, T* Yc,
-                      int B, hipStream_t s) {
+                      int B, const GemmShape& shape, hipStream_t s) {
   if (B < 1 || B > kMaxCols) throw Error(IDFT_EINVAL, "block width out of range");
   switch (g.p) {
-    case 2: cell_gemm_p<T, 2>(g, H, S, X, xcol, Yc, B, s); break;
-    case 3: cell_gemm_p<T, 3>(g, H, S, X, xcol, Yc, B, s); break;
-    case 4: cell_gemm_p<T, 4>(g, H, S, X, xcol, Yc, B, s); break;
-    case 5: cell_gemm_p<T, 5>(g, H, S, X, xcol, Yc, B, s); break;
+    case 2: cell_gemm_p<T, 2>(g, H, S, X, xcol, Yc, B, shape, s); break;
+    case 3: cell_gemm_p<T, 3>(g, H, S, X, xcol, Yc, B, shape, s); break;
+    case 4: cell_gemm_p<T, 4>(g, H, S, X, xcol, Yc, B, shape, s); break;
+    case 5: cell_gemm_p<T, 5>(g, H, S, X, xcol, Yc, B, shape, s); break;
     default: throw Error(IDFT_EINVAL, "degree not supported by the HIP kernels (2..5)");
   }
   IDFT_HIP(hipGetLastError());
 }
 template void launch_cell_gemm<float>(const Geo&, const float*, const float*, const float*,
-                                      const double*, float*, int, hipStream_t);
+                                      const double*, float*, int, const GemmShape&, hipStream_t);
 template void launch_cell_gemm<double>(const Geo&, const double*, const double*, const double*,
-                                       const double*, double*, int, hipStream_t);
+                                       const double*, double*, int, const GemmShape&, hipStream_t);
 
 template <typename T>
 int launch_cell_apply(const Geo& g, const T* H, const T* S, const T* X, T* Yc, int B,
-                      const ApplyEpilogue<T>& ep, hipStream_t s) {
+                      const ApplyEpilogue<T>& ep, const GemmShape& shape, hipStream_t s) {
   if (B < 1 || B > kMaxCols) throw Error(IDFT_EINVAL, "block width out of range");
   switch (g.p) {
-    case 2: cell_gemm_p<T, 2>(g, H, S, X, ep.xcol, Yc, B, s); break;
-    case 3: cell_gemm_p<T, 3>(g, H, S, X, ep.xcol, Yc, B, s); break;
-    case 4: cell_gemm_p<T, 4>(g, H, S, X, ep.xcol, Yc, B, s); break;
-    case 5: cell_gemm_p<T, 5>(g, H, S, X, ep.xcol, Yc, B, s); break;
+    case 2: cell_gemm_p<T, 2>(g, H, S, X, ep.xcol, Yc, B, shape, s); break;
+    case 3: cell_gemm_p<T, 3>(g, H, S, X, ep.xcol, Yc, B, shape, s); break;
+    case 4: cell_gemm_p<T, 4>(g, H, S, X, ep.xcol, Yc, B, shape, s); break;
+    case 5: cell_gemm_p<T, 5>(g, H, S, X, ep.xcol, Yc, B, shape, s); break;
     default: throw Error(IDFT_EINVAL, "degree not supported by the HIP kernels (2..5)");
   }
   IDFT_HIP(hipGetLastError());
   check_u32(g.N * B);
-  const int nb = gather_blocks(g.N, B);
-  k_cell_gather<T><<<nb, kThreads, 0, s>>>(g, (uint32_t)B, Yc, S, ep.Y, ep.alpha, ep.beta, ep.X,
-                                           ep.a, ep.ca, ep.Z, ep.b, ep.cb, ep.W, ep.c, ep.cc, X,
-                                           ep.xcol, ep.V, ep.dot_partial);
+  unsigned gx, gy;
+  uint32_t R;
+  gather_grid(g, B, gx, gy, R);
+  dim3 grid(gx, gy);
+#define IDFT_GATHER(PP)                                                                       \
+  k_cell_gather<T, PP><<<grid, kThreads, 0, s>>>(g, (uint32_t)B, R, Yc, S, ep.Y, ep.alpha,   \
+                                                 ep.beta, ep.X, ep.a, ep.ca, ep.Z, ep.b, ep.cb, \
+                                                 ep.W, ep.c, ep.cc, X, ep.xcol, ep.V,         \
+                                                 ep.dot_partial)
+  switch (g.p) {
+    case 2: IDFT_GATHER(2); break;
+    case 3: IDFT_GATHER(3); break;
+    case 4: IDFT_GATHER(4); break;
+    case 5: IDFT_GATHER(5); break;
+  }
+#undef IDFT_GATHER
   IDFT_HIP(hipGetLastError());
-  return nb;
+  return (int)(gx * gy);
 }
 template int launch_cell_apply<float>(const Geo&, const float*, const float*, const float*, float*,
-                                      int, const ApplyEpilogue<float>&, hipStream_t);
+                                      int, const ApplyEpilogue<float>&, const GemmShape&,
+                                      hipStream_t);
 template int launch_cell_apply<double>(const Geo&, const double*, const double*, const double*,
-                                       double*, int, const ApplyEpilogue<double>&, hipStream_t);
+                                       double*, int, const ApplyEpilogue<double>&,
+                                       const GemmShape&, hipStream_t);
 
 template <typename T>
 void launch_lincomb(int64_t N, int B, T* Y, double beta, const T* X, double a, const double* ca,
@@ -1324,7 +1377,7 @@ void launch_minres_givens(int B, int nblk, const double* partial, const double* 
   IDFT_HIP(hipGetLastError());
 }
 template <typename T>
-void launch_minres_update(int64_t N, int B, const T* v, T* w1, const T* w, double* x,
+void launch_minres_update(int64_t N, int B, const T* v, T* w1, const T* w, T* x,
                           const double* st, hipStream_t s) {
   check_u32(N * B);
   k_minres_update<T><<<grid_for(N * B), kThreads, 0, s>>>((uint32_t)(N * B), (uint32_t)B, v, w1, w,
@@ -1344,7 +1397,7 @@ template void launch_minres_prep<float>(int64_t, int, float*, float*, const floa
 template void launch_minres_prep<double>(int64_t, int, double*, double*, const double*,
                                          const double*, const double*, hipStream_t);
 template void launch_minres_update<float>(int64_t, int, const float*, float*, const float*,
-                                          double*, const double*, hipStream_t);
+                                          float*, const double*, hipStream_t);
 template void launch_minres_update<double>(int64_t, int, const double*, double*, const double*,
                                            double*, const double*, hipStream_t);
 void launch_copy_done(int B, const double* st, int* flags, hipStream_t s) {

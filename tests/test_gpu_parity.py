@@ -162,3 +162,20 @@ def test_bfgs_two_iterations(glib, olib):
     assert np.allclose(lg, lo, rtol=1e-7, atol=0)
     assert np.allclose(gg, go, rtol=1e-6, atol=0)
     assert lg[-1] < lg[0]
+
+
+@pytest.mark.parametrize("p,ncols", [(4, 40), (5, 12), (3, 80)])
+def test_every_tile_shape_keeps_parity(glib, olib, p, ncols):
+    """Every cell-product tile shape on the autotuner's grid gives the same operator."""
+    _say("tile_shapes", p, ncols)
+    g, o, s = bare_pair(glib, olib, 3, p, 4, precision=abi.MIXED)
+    x = np.random.default_rng(7).standard_normal((s.n_nodes, ncols))
+    yo = o.apply_hamiltonian(x)
+    for cp in (1, 2, 4, 8):
+        for vt in (16, 32, 64, 80):
+            for prec, tol in ((abi.FP64, 1e-13), (abi.MIXED, 2e-6)):
+                g.set_apply_shape(prec, cp, vt)
+                assert rel(g.apply_hamiltonian(x, precision=prec), yo) < tol, (cp, vt, prec)
+    best = g.autotune_apply(ncols, abi.FP64, reps=2)
+    assert best["cells_per_wg"] in (1, 2, 4, 8) and best["vectors_per_tile"] % 16 == 0
+    assert rel(g.apply_hamiltonian(x), yo) < 1e-13
