@@ -134,7 +134,7 @@ struct ApplyEpilogue {
   T* V = nullptr;
   double* dot_partial = nullptr;
 };
-constexpr int kGatherBlocks = 2048;
+constexpr int kGatherBlocks = 16384;
 int gather_blocks(const Geo& g, int B);  // rows of the gather's dot partials
 // Cell operator in two launches: per-cell H_e (S xcol Xin)_e into the scratch
 // Yc (cell_scratch_elems(g, B) elements), then the node gather + epilogue.
