@@ -29,6 +29,11 @@ sys.path.insert(0, HERE)
 
 METRIC = "s per BFGS iter (eigen+adjoint+force), mixed vs FP64 on GPU, force err ≤1e-6"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the dominant kernel from two rocprofv3 --pmc runs
+# (FETCH_SIZE, WRITE_SIZE) of `bench.py --kernels-only` at the autotuned
+# shape, FETCH_SIZE scaled by the copy calibration; the summary with the
+# commands is profiles/cell_gemm_inverse_24c_p5_pmc.md.
+MEASURED_TRAFFIC = {}
 
 
 def log(*a):
@@ -94,6 +99,40 @@ def cpu_baseline(prob, per_eval, evals_per_iter):
                        f"iteration")}
 
 
+def kernels_only(lib, cfg, shape, reps):
+    """Operator kernels alone, for hardware-counter runs: the cell operator of
+    the workload's mesh (soft-Coulomb potential) applied `reps` times to a
+    block of n_eig columns in FP32 and FP64 at a fixed launch shape, after
+    three Jacobi-preconditioner calls that stream a known number of bytes
+    (read N*B*8 + N*8, write N*B*8) to calibrate the byte counters."""
+    from invdft_amd import abi, synthetic
+    d = abi.default_desc(lib, n_cells=cfg["n_cells"], degree=cfg["degree"],
+                         half_width=cfg["half_width"], n_eig=cfg["n_eig"],
+                         n_electrons=cfg["n_electrons"], precision=abi.MIXED)
+    s = abi.geometry(lib, d)
+    atoms = cfg["atoms"] if cfg["atoms"] != "random8" else synthetic.random_atoms(1)
+    v = synthetic.soft_coulomb(abi.quad_points(lib, d), atoms)
+    nq = s.n_cells * s.quad_per_cell
+    ctx = abi.Context(lib, d, v, np.zeros(nq), np.ones(nq), np.ones(s.n_nodes))
+    ctx.set_veff_quad(v)
+    B = cfg["n_eig"]
+    a = ctx.mv(B, np.random.default_rng(0).standard_normal((s.n_nodes, B)))
+    b = ctx.mv(B)
+    for _ in range(3):
+        ctx.adj_precondition_jacobi(b, a)
+    log(f"copy calibration: 3 x (read {s.n_nodes * B * 8 + s.n_nodes * 8} B, "
+        f"write {s.n_nodes * B * 8} B)")
+    out = {}
+    for name, prec in (("fp32", abi.MIXED), ("fp64", abi.FP64)):
+        ctx.set_apply_shape(prec, *shape)
+        out[name] = ctx.profile_apply(B, prec, reps=reps)
+        log(f"{name} shape {shape}: operator {out[name][0]:.4f} ms, cell product "
+            f"{out[name][1]:.4f} ms")
+    ctx.close()
+    print(json.dumps({"kernels_only": out, "shape": shape, "n_nodes": int(s.n_nodes)}),
+          flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -103,6 +142,11 @@ def main():
     ap.add_argument("--cheb-tol", type=float, default=1e-8)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="only the operator kernels at --shape (for counter runs)")
+    ap.add_argument("--shape", type=int, nargs=3, default=[1, 16, 0],
+                    metavar=("CELLS_PER_WG", "VECTORS_PER_TILE", "GATHER_ROWS"))
+    ap.add_argument("--reps", type=int, default=5)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (--gpus {args.gpus} requested); the multi-GPU "
@@ -114,6 +158,9 @@ def main():
 
     lib = invdft_amd.load()
     cfg = dict(synthetic.CONFIGS[args.config])
+    if args.kernels_only:
+        kernels_only(lib, cfg, args.shape, args.reps)
+        return
     log(f"config {args.config}: {cfg}")
     t0 = time.perf_counter()
     prob = synthetic.make_problem(lib, seed=2, target_tol=1e-10, **cfg)
@@ -146,27 +193,33 @@ def main():
 
     # roofline of the dominant kernel: the FP32 MFMA cell-product kernel
     # k_cell_gemm (phase 1 of the operator).  Algorithmic bytes per launch: the
-    # stored H_e read once, the block X read once, and the per-cell products
-    # written once (the gather kernel reads them back).
+    # symmetric H_e (ne(ne+1)/2 entries per cell) read once, the block X and
+    # the node scaling S read once, and the per-cell products written once
+    # (the gather kernel reads them back).
     s = cm.sizes
     B = prob.desc.n_eig
     ne = s.nodes_per_cell
+    nh = ne * (ne + 1) // 2
+
+    def gemm_bytes(sz):
+        return s.n_cells * nh * sz + s.n_nodes * (B + 1) * sz + s.n_cells * ne * B * sz
+
     ms_op, ms_gemm = cm.profile_apply(B, abi.MIXED, reps=20)
-    bytes_gemm = s.n_cells * ne * ne * 4 + s.n_nodes * B * 4 + s.n_cells * ne * B * 4
-    bytes_op = s.n_cells * ne * ne * 4 + 2 * s.n_nodes * B * 4
+    bytes_gemm = gemm_bytes(4)
+    bytes_op = s.n_cells * nh * 4 + 2 * s.n_nodes * B * 4
     achieved = bytes_gemm / (ms_gemm * 1e-3) / 1e9
+    kname = f"k_cell_gemm<float,{prob.desc.degree},{(B + 15) // 16}>"
+    traffic = MEASURED_TRAFFIC.get((args.config, kname))
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": f"k_cell_gemm<float,{prob.desc.degree},{(B + 15) // 16}>",
-            "ms_per_launch": round(ms_gemm, 4), "bytes_per_launch": bytes_gemm,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "kernel": kname, "ms_per_launch": round(ms_gemm, 4), "bytes_per_launch": bytes_gemm,
             "operator_ms": round(ms_op, 4),
             "operator_GBs": round(bytes_op / (ms_op * 1e-3) / 1e9, 1)}
     extra = {}
     if cf is not None:
         m64, g64 = cf.profile_apply(B, abi.FP64, reps=20)
-        b64 = s.n_cells * ne * ne * 8 + s.n_nodes * B * 8 + s.n_cells * ne * B * 8
         extra["fp64_gemm"] = {"ms_per_launch": round(g64, 4), "operator_ms": round(m64, 4),
-                              "achieved_GBs": round(b64 / (g64 * 1e-3) / 1e9, 1)}
+                              "achieved_GBs": round(gemm_bytes(8) / (g64 * 1e-3) / 1e9, 1)}
     cpu = None
     if not args.no_cpu:
         log("CPU baseline sample (oracle)")

@@ -90,6 +90,11 @@ typedef struct idft_desc {
   int32_t refine_every;       /* MIXED: inner MINRES steps per FP64 refinement  */
   int32_t check_every;        /* MINRES steps between host convergence checks   */
   uint64_t seed;              /* seed of the initial random subspace            */
+  int32_t minres_graph;       /* HIP: replay check_every MINRES steps as one
+                                 captured HIP graph (1) or launch each kernel
+                                 (0); same results either way                  */
+  double inner_tol;           /* MIXED: FP32 inner MINRES stops at this factor
+                                 times the FP64 refinement residual            */
 } idft_desc;
 
 typedef struct idft_sizes {

@@ -5,6 +5,7 @@ product package loads only its own HIP library (invdft_amd/_lib/libidft.so,
 see ``invdft_amd.load``); the test suite binds the CPU oracle separately.
 """
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -29,7 +30,7 @@ class Desc(C.Structure):
         ("additional_states", C.c_int32), ("block_size", C.c_int32),
         ("rho_tol_constraints", C.c_double), ("gradient_mode", C.c_int32),
         ("lanczos_steps", C.c_int32), ("refine_every", C.c_int32), ("check_every", C.c_int32),
-        ("seed", C.c_uint64),
+        ("seed", C.c_uint64), ("minres_graph", C.c_int32), ("inner_tol", C.c_double),
     ]
 
 
@@ -177,6 +178,51 @@ def basis_tables(lib, p):
     if rc:
         raise IdftError(rc, "invalid degree")
     return gx, gw, qx, qw
+
+
+# include/idft_io.h (HIP library only)
+_IO_PROTOS = {
+    "idft_write_vxc_text": (C.c_int, [C.POINTER(Desc), C.c_char_p, _DP, C.c_int32]),
+    "idft_read_vxc_text": (C.c_int, [C.POINTER(Desc), C.c_char_p, C.c_double, C.c_int32, _DP]),
+    "idft_write_vxc_binary": (C.c_int, [C.POINTER(Desc), C.c_char_p, _DP, C.c_int32]),
+    "idft_read_vxc_binary": (C.c_int, [C.POINTER(Desc), C.c_char_p, C.c_int32, _DP]),
+    "idft_set_vxc_output": (C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p, C.c_int32]),
+    "idft_bfgs_set_checkpoint": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int32, C.c_int32]),
+}
+
+
+def _io(lib, name):
+    fn = getattr(lib, name)
+    fn.restype, fn.argtypes = _IO_PROTOS[name]
+    return fn
+
+
+def _path(p):
+    return os.fsencode(p)
+
+
+def write_vxc(lib, desc, path, pot, binary=False):
+    """Write v_xc (n_vxc values, or (n_spin, n_vxc)) as text records or binary."""
+    pot = _f64(pot)
+    n_spin = 1 if pot.ndim == 1 else pot.shape[0]
+    name = "idft_write_vxc_binary" if binary else "idft_write_vxc_text"
+    rc = _io(lib, name)(C.byref(desc), _path(path), _dp(pot), n_spin)
+    if rc:
+        raise IdftError(rc, f"cannot write {path}")
+
+
+def read_vxc(lib, desc, path, n_spin=1, binary=False, dist_tol=1e-3):
+    """Read a v_xc file written by write_vxc (or by the reference's text writer)."""
+    n = geometry(lib, desc).n_vxc
+    pot = np.zeros(n_spin * n)
+    if binary:
+        rc = _io(lib, "idft_read_vxc_binary")(C.byref(desc), _path(path), n_spin, _dp(pot))
+    else:
+        rc = _io(lib, "idft_read_vxc_text")(C.byref(desc), _path(path), dist_tol, n_spin,
+                                            _dp(pot))
+    if rc:
+        raise IdftError(rc, f"cannot read {path}")
+    return pot if n_spin == 1 else pot.reshape(n_spin, n)
 
 
 class MultiVector:
@@ -374,6 +420,16 @@ class Context:
                                                iters.ctypes.data_as(_IP)))
         return iters
 
+    def set_vxc_output(self, folder, postfix, frequency):
+        """Write every frequency-th evaluation's input v_xc as text records."""
+        self._check(_io(self.lib, "idft_set_vxc_output")(self.h, _path(folder), _path(postfix),
+                                                          frequency))
+
+    def set_bfgs_checkpoint(self, path, every=1, resume=True):
+        """L-BFGS state file written every `every` iterations (None: off)."""
+        self._check(_io(self.lib, "idft_bfgs_set_checkpoint")(
+            self.h, None if path is None else _path(path), every, int(resume)))
+
     def bfgs_solve(self, ls_type=LS_CP, max_iter=10, tol=1e-12, ls_tol=1e-6, history=100,
                    n_line_search=1):
         lh = np.zeros(max_iter + 1)
@@ -386,21 +442,23 @@ class Context:
         self.bfgs_times_ms = th[: nd.value + 1]
         return lh[: nd.value + 1], gh[: nd.value + 1], nd.value
 
-    def set_apply_shape(self, precision, cells_per_wg, vectors_per_tile):
+    def set_apply_shape(self, precision, cells_per_wg, vectors_per_tile, gather_rows=0):
         fn = self.lib.idft_set_apply_shape
         fn.restype = C.c_int
-        fn.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]
-        self._check(fn(self.h, precision, cells_per_wg, vectors_per_tile))
+        fn.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32]
+        self._check(fn(self.h, precision, cells_per_wg, vectors_per_tile, gather_rows))
 
     def autotune_apply(self, ncols, precision=FP64, reps=5):
-        """Pick the fastest cell-product tile shape (HIP library only)."""
-        cp, vt = C.c_int32(0), C.c_int32(0)
+        """Pick the fastest operator launch shape (HIP library only)."""
+        cp, vt, gr = C.c_int32(0), C.c_int32(0), C.c_int32(0)
         ms = np.zeros(1)
         fn = self.lib.idft_autotune_apply
         fn.restype = C.c_int
-        fn.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, _IP, _IP, _DP]
-        self._check(fn(self.h, ncols, precision, reps, C.byref(cp), C.byref(vt), _dp(ms)))
-        return dict(cells_per_wg=cp.value, vectors_per_tile=vt.value, ms=float(ms[0]))
+        fn.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, _IP, _IP, _IP, _DP]
+        self._check(fn(self.h, ncols, precision, reps, C.byref(cp), C.byref(vt), C.byref(gr),
+                       _dp(ms)))
+        return dict(cells_per_wg=cp.value, vectors_per_tile=vt.value, gather_rows=gr.value,
+                    ms=float(ms[0]))
 
     def profile_apply(self, ncols, precision=FP64, reps=10):
         """Device ms per Hamiltonian application and for its MFMA cell-product

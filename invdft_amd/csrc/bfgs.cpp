@@ -5,6 +5,8 @@
 // doubles (the reference keeps them as host deal.II vectors too).
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstring>
 #include <deque>
 #include <string>
 #include <vector>
@@ -14,6 +16,8 @@
 namespace {
 
 using Vec = std::vector<double>;
+
+const char kMagic[8] = {'I', 'D', 'F', 'T', 'L', 'B', 'F', '1'};
 
 class Lbfgs {
  public:
@@ -25,16 +29,23 @@ class Lbfgs {
     auto elapsed = [&] {
       return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     };
+    const idft::BfgsCheckpoint& ck = idft::bfgs_checkpoint_of(c_);
     x_.assign(n_, 0.0);
-    int rc = idft_get_initial_guess(c_, x_.data());
-    if (rc) return rc;
     g_.assign(n_, 0.0);
     p_.assign(n_, 0.0);
     Vec y(n_, 0.0), s(n_, 0.0);
     double L = 0.0;
+    bool resumed = false;
+    int rc = IDFT_OK;
+    if (ck.resume && !ck.path.empty()) {
+      rc = load(ck.path, L);
+      if (rc == IDFT_OK) resumed = true;
+      else if (rc != kNoFile) return rc;
+    }
+    if (!resumed && (rc = idft_get_initial_guess(c_, x_.data()))) return rc;
     int iter = 0;
     for (; iter < maxIter; ++iter) {
-      if (iter == 0 && (rc = evaluate(x_, g_, L))) return rc;
+      if (iter == 0 && !resumed && (rc = evaluate(x_, g_, L))) return rc;
       const double gn = std::sqrt(dot(g_, g_));
       if (timeHist) timeHist[iter] = elapsed();
       if (lossHist) lossHist[iter] = L;
@@ -69,6 +80,7 @@ class Lbfgs {
         S_.pop_front();
         rho_.pop_front();
       }
+      if (!ck.path.empty() && (iter + 1) % ck.every == 0 && (rc = save(ck.path, L))) return rc;
     }
     if (iter == maxIter) {
       if (lossHist) lossHist[iter] = L;
@@ -81,6 +93,66 @@ class Lbfgs {
   }
 
  private:
+  static constexpr int kNoFile = 1;
+
+  // Checkpoint: "IDFTLBF1", int64 n, int32 m, double L, x[n], g[n], then m
+  // records (s[n], y[n], rho), oldest first.  Written to path + ".tmp" and
+  // renamed, so a reader never sees a partial file.
+  int save(const std::string& path, double L) const {
+    const std::string tmp = path + ".tmp";
+    std::FILE* f = std::fopen(tmp.c_str(), "wb");
+    if (!f) return IDFT_EINVAL;
+    const int64_t n = n_;
+    const int32_t m = (int32_t)Y_.size();
+    bool ok = std::fwrite(kMagic, 1, 8, f) == 8 && std::fwrite(&n, sizeof n, 1, f) == 1 &&
+              std::fwrite(&m, sizeof m, 1, f) == 1 && std::fwrite(&L, sizeof L, 1, f) == 1 &&
+              std::fwrite(x_.data(), sizeof(double), n_, f) == (size_t)n_ &&
+              std::fwrite(g_.data(), sizeof(double), n_, f) == (size_t)n_;
+    for (int32_t k = 0; ok && k < m; ++k)
+      ok = std::fwrite(S_[k].data(), sizeof(double), n_, f) == (size_t)n_ &&
+           std::fwrite(Y_[k].data(), sizeof(double), n_, f) == (size_t)n_ &&
+           std::fwrite(&rho_[k], sizeof(double), 1, f) == 1;
+    ok = (std::fclose(f) == 0) && ok;
+    if (!ok || std::rename(tmp.c_str(), path.c_str()) != 0) return IDFT_EINVAL;
+    return IDFT_OK;
+  }
+
+  int load(const std::string& path, double& L) {
+    std::FILE* f = std::fopen(path.c_str(), "rb");
+    if (!f) return kNoFile;
+    char magic[8];
+    int64_t n = 0;
+    int32_t m = 0;
+    bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, kMagic, 8) == 0 &&
+              std::fread(&n, sizeof n, 1, f) == 1 && n == n_ &&
+              std::fread(&m, sizeof m, 1, f) == 1 && m >= 0 && m <= history_ + 1 &&
+              std::fread(&L, sizeof L, 1, f) == 1 &&
+              std::fread(x_.data(), sizeof(double), n_, f) == (size_t)n_ &&
+              std::fread(g_.data(), sizeof(double), n_, f) == (size_t)n_;
+    Y_.clear();
+    S_.clear();
+    rho_.clear();
+    for (int32_t k = 0; ok && k < m; ++k) {
+      Vec sv(n_), yv(n_);
+      double r = 0.0;
+      ok = std::fread(sv.data(), sizeof(double), n_, f) == (size_t)n_ &&
+           std::fread(yv.data(), sizeof(double), n_, f) == (size_t)n_ &&
+           std::fread(&r, sizeof(double), 1, f) == 1;
+      if (ok) {
+        S_.push_back(std::move(sv));
+        Y_.push_back(std::move(yv));
+        rho_.push_back(r);
+      }
+    }
+    std::fclose(f);
+    while ((int)Y_.size() > history_) {
+      Y_.pop_front();
+      S_.pop_front();
+      rho_.pop_front();
+    }
+    return ok ? IDFT_OK : IDFT_EINVAL;
+  }
+
   double dot(const Vec& a, const Vec& b) {
     double o = 0.0;
     idft_dot(c_, a.data(), b.data(), n_, 1, &o);

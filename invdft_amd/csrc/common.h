@@ -10,6 +10,7 @@
 
 #include "../../include/idft.h"
 #include "../../include/idft_hip.h"
+#include "../../include/idft_io.h"
 
 namespace idft {
 
@@ -118,11 +119,11 @@ int64_t hamiltonian_tile_elems(int p);
 void launch_build_hamiltonian(const Geo& g, const DevTables& t, double hh, const double* veff,
                               const double* jxw, double* h64, float* h32, hipStream_t s);
 // Output combination of one operator application:
-//   Y = beta Y + alpha S[n] (sum_e P_e^T H_e P_e S X)[n] + a ca X + b cb Z + c cc W
-// (null coefficient arrays read as 1, null blocks as 0).
-// The operator input is xcol[j] Xin (xcol may be null); optionally V receives
-// that scaled input and dot_partial[blk][j] the block partials of V.Y
-// (gather_blocks(N, B) blocks).
+//   Y = beta Y + alpha S[n] (sum_e P_e^T H_e P_e S V)[n] + a ca xn X + b cb Z + c cc W
+// (null coefficient arrays read as 1, null blocks as 0), where the operator
+// input is V = xcol[j] xn[n] Xin (xcol, xn may be null).  With dot_partial,
+// dot_partial[blk][j] receives the block partials of V.Y (gather_blocks
+// blocks) and V, if given, the scaled input.
 template <typename T>
 struct ApplyEpilogue {
   T* Y = nullptr;
@@ -130,12 +131,13 @@ struct ApplyEpilogue {
   const T* X = nullptr; double a = 0.0; const double* ca = nullptr;
   const T* Z = nullptr; double b = 0.0; const double* cb = nullptr;
   const T* W = nullptr; double c = 0.0; const double* cc = nullptr;
-  const double* xcol = nullptr;
-  T* V = nullptr;
+  const double* xn = nullptr;    // node scale of the operator input and of X
+  const double* xcol = nullptr;  // column scale of the operator input
+  T* V = nullptr;                // stores the scaled input xcol xn Xin (needs dot_partial)
   double* dot_partial = nullptr;
 };
 constexpr int kGatherBlocks = 16384;
-int gather_blocks(const Geo& g, int B);  // rows of the gather's dot partials
+int gather_blocks(const Geo& g, int B, int rows);  // rows of the gather's dot partials
 // Cell operator in two launches: per-cell H_e (S xcol Xin)_e into the scratch
 // Yc (cell_scratch_elems(g, B) elements), then the node gather + epilogue.
 // Returns the number of gather blocks (rows of dot_partial).
@@ -143,6 +145,7 @@ int gather_blocks(const Geo& g, int B);  // rows of the gather's dot partials
 struct GemmShape {
   int cells_per_wg = 1;       // cells processed in sequence by one work-group
   int vectors_per_tile = 80;  // columns per launch tile (rounded to 16; >= B: one pass)
+  int gather_rows = 0;        // node rows per gather work-group (0: about kGatherBlocks groups)
 };
 template <typename T>
 int launch_cell_apply(const Geo& g, const T* H, const T* S, const T* Xin, T* Yc, int B,
@@ -213,16 +216,21 @@ void launch_minres_start(int B, int nblk, const double* partial, const double* n
                          double* st, hipStream_t s);
 void launch_minres_alpha(int B, int nblk, const double* partial, double* st, hipStream_t s);
 template <typename T>
-int launch_minres_lanczos(int64_t N, int B, T* y, const T* r2, T* rnew, const double* D,
+int launch_minres_lanczos(int64_t N, int B, const T* y, const T* r2, T* rnew, const double* D,
                           const double* st, double* partial, hipStream_t s);
 void launch_minres_givens(int B, int nblk, const double* partial, const double* negEig,
                           double* st, hipStream_t s);
 template <typename T>
-void launch_minres_update(int64_t N, int B, const T* v, T* w1, const T* w, T* x,
+void launch_minres_update(int64_t N, int B, const T* r, const double* D, T* w1, const T* w, T* x,
                           const double* st, hipStream_t s);
 void launch_copy_done(int B, const double* st, int* flags, hipStream_t s);
-template <typename T>
-void launch_minres_prep(int64_t N, int B, T* y, T* v, const T* r1, const double* negEig,
-                        const double* st, hipStream_t s);
+
+// L-BFGS checkpoint settings of a context (idft_bfgs_set_checkpoint).
+struct BfgsCheckpoint {
+  std::string path;  // empty: off
+  int every = 1;
+  bool resume = false;
+};
+const BfgsCheckpoint& bfgs_checkpoint_of(const idft_ctx* c);
 
 }  // namespace idft

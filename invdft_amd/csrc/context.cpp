@@ -5,6 +5,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <set>
@@ -43,9 +45,9 @@ struct AdjointState {
 // Work buffers of one MINRES precision.
 template <typename T>
 struct MinresWork {
-  DevBuf<T> r1, r2, y, v, w1, w, rhs;
+  DevBuf<T> r1, r2, y, w1, w, rhs;
   void ensure(size_t n) {
-    r1.resize(n); r2.resize(n); y.resize(n); v.resize(n); w1.resize(n); w.resize(n); rhs.resize(n);
+    r1.resize(n); r2.resize(n); y.resize(n); w1.resize(n); w.resize(n); rhs.resize(n);
   }
 };
 
@@ -61,6 +63,7 @@ struct idft_ctx {
   int B = 0;
   int64_t nQ = 0;
   bool mixed = false;
+  bool verbose = false;  // IDFT_VERBOSE set: solver progress on stderr
   // static per-node / per-quadrature data
   DevBuf<double> S, sqrtM, invSqrtM, Djac, DjacSqrt, Mdiag, jxw;
   DevBuf<float> S32;
@@ -86,6 +89,11 @@ struct idft_ctx {
   double tolCheb = 1e-8, adjointTol = 1e-11, lossPrev = 0.0, loss = 0.0;
   bool haveForward = false;
   std::vector<double> pot0, potSol;
+  // v_xc output of getForceVector and L-BFGS checkpoint settings (idft_io.h)
+  std::string vxcFolder, vxcPostfix;
+  int vxcFreq = 0;
+  std::vector<double> potEval;  // input potential of the current evaluation
+  BfgsCheckpoint ckpt;
   idft_stats st{};
   AdjointState adj;
   MinresWork<double> mw64;
@@ -632,39 +640,88 @@ struct idft_ctx {
     const int64_t N = t.N;
     launch_lincomb<T>(N, b, W.r1.p, 0.0, rhs, 1.0, nullptr, nullptr, 0.0, nullptr, nullptr, s);
     launch_lincomb<T>(N, b, W.r2.p, 0.0, rhs, 1.0, nullptr, nullptr, 0.0, nullptr, nullptr, s);
-    launch_lincomb<T>(N, b, W.y.p, 0.0, rhs, 1.0, nullptr, nullptr, 0.0, nullptr, Djac.p, s);
     launch_lincomb<T>(N, b, W.w1.p, 0.0, nullptr, 0.0, nullptr, nullptr, 0.0, nullptr, nullptr, s);
     launch_lincomb<T>(N, b, W.w.p, 0.0, nullptr, 0.0, nullptr, nullptr, 0.0, nullptr, nullptr, s);
-    partial.resize((size_t)std::max(1024, gather_blocks(g, b)) * b);
-    int nb = launch_coldot_partial<T>(N, b, W.r1.p, W.y.p, nullptr, partial.p, s);
+    const GemmShape& shp = std::is_same<T, double>::value ? shape64 : shape32;
+    partial.resize((size_t)std::max(1024, gather_blocks(g, b, shp.gather_rows)) * b);
+    int nb = launch_coldot_partial<T>(N, b, W.r1.p, W.r1.p, Djac.p, partial.p, s);
     launch_minres_start(b, nb, partial.p, adj.dnegEig.p, mst.p, s);
     const int check = std::max(1, d.check_every);
     std::vector<int> flags(b);
+    // r2 = r_k (newest Lanczos residual), r1 = r_{k-1}; the step's Lanczos
+    // vector v = cv D r2 is formed on the fly (not stored)
     T *r1 = W.r1.p, *r2 = W.r2.p, *w1 = W.w1.p, *w = W.w.p;
-    for (int k = 0; k < maxit; ++k) {
-      if (k % check == 0) {
-        launch_copy_done(b, mst.p, mflags.p, s);
-        IDFT_HIP(hipMemcpyAsync(flags.data(), mflags.p, b * sizeof(int), hipMemcpyDeviceToHost, s));
-        sync();
-        bool all = true;
-        for (int f : flags) all = all && f;
-        if (all) break;
-      }
-      // v = cv y;  y <- H~ v - eps v + cr1 r1;  partial dots of v.y  (one apply)
+    // one MINRES step: 5 launches, all arguments device-resident (scalars in
+    // mst, done flags per column), so a run of steps can be replayed as a
+    // graph.  The two buffer swaps return to the start after two steps.
+    // N*B streams per step: 1 (cell product) + 3 (gather) + 3 (Lanczos) + 6
+    // (update).
+    auto step = [&] {
+      // y <- H~ v - eps v + cr1 r1, v = cv D r2;  partial dots of v.y  (one apply)
       ApplyEpilogue<T> ep;
       ep.Y = W.y.p; ep.alpha = 1.0;
-      ep.X = W.y.p; ep.a = 1.0; ep.ca = mst.p + (size_t)MS_CVN * b;
+      ep.X = r2; ep.a = 1.0; ep.ca = mst.p + (size_t)MS_CVN * b;
       ep.Z = r1; ep.b = 1.0; ep.cb = mst.p + (size_t)MS_CR1 * b;
+      ep.xn = Djac.p;
       ep.xcol = mst.p + (size_t)MS_CV * b;
-      ep.V = W.v.p;
       ep.dot_partial = partial.p;
-      nb = apply<T>(W.y.p, b, ep);
-      launch_minres_alpha(b, nb, partial.p, mst.p, s);
-      nb = launch_minres_lanczos<T>(N, b, W.y.p, r2, r1, Djac.p, mst.p, partial.p, s);
-      launch_minres_givens(b, nb, partial.p, adj.dnegEig.p, mst.p, s);
-      launch_minres_update<T>(N, b, W.v.p, w1, w, x, mst.p, s);
+      int nbk = apply<T>(r2, b, ep);
+      launch_minres_alpha(b, nbk, partial.p, mst.p, s);
+      nbk = launch_minres_lanczos<T>(N, b, W.y.p, r2, r1, Djac.p, mst.p, partial.p, s);
+      launch_minres_givens(b, nbk, partial.p, adj.dnegEig.p, mst.p, s);
+      launch_minres_update<T>(N, b, r2, Djac.p, w1, w, x, mst.p, s);
       std::swap(r1, r2);
       std::swap(w1, w);
+    };
+    // check_every steps between host convergence checks, captured once per
+    // solve as a HIP graph and replayed (kernels only: no memset or copy
+    // nodes); the apply scratch is sized before the capture.
+    const bool useGraph = d.minres_graph != 0 && check % 2 == 0;
+    if constexpr (std::is_same<T, double>::value) Yc64.resize(cell_scratch_elems(g, b));
+    else Yc32.resize(cell_scratch_elems(g, b));
+    hipGraphExec_t exec = nullptr;
+    struct ExecGuard {
+      hipGraphExec_t& e;
+      ~ExecGuard() { if (e) (void)hipGraphExecDestroy(e); }
+    } guard{exec};
+    for (int k = 0; k < maxit;) {
+      launch_copy_done(b, mst.p, mflags.p, s);
+      IDFT_HIP(hipMemcpyAsync(flags.data(), mflags.p, b * sizeof(int), hipMemcpyDeviceToHost, s));
+      sync();
+      bool all = true;
+      for (int f : flags) all = all && f;
+      if (all) break;
+      const int n = std::min(check, maxit - k);
+      if (useGraph && n == check) {
+        if (!exec) {
+          hipGraph_t graph = nullptr;
+          IDFT_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+          try {
+            for (int i = 0; i < n; ++i) step();
+          } catch (...) {
+            (void)hipStreamEndCapture(s, &graph);
+            if (graph) (void)hipGraphDestroy(graph);
+            throw;
+          }
+          IDFT_HIP(hipStreamEndCapture(s, &graph));
+          const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+          (void)hipGraphDestroy(graph);
+          IDFT_HIP(ie);
+        } else {
+          // the capture counted its n steps; replays count theirs here
+          if constexpr (std::is_same<T, double>::value) {
+            st.applies_fp64 += n;
+            st.apply_cols_fp64 += (int64_t)n * b;
+          } else {
+            st.applies_fp32 += n;
+            st.apply_cols_fp32 += (int64_t)n * b;
+          }
+        }
+        IDFT_HIP(hipGraphLaunch(exec, s));
+      } else {
+        for (int i = 0; i < n; ++i) step();
+      }
+      k += n;
     }
     std::vector<double> its(b);
     IDFT_HIP(hipMemcpyAsync(its.data(), mst.p + (size_t)MS_IT * b, b * sizeof(double),
@@ -693,9 +750,13 @@ struct idft_ctx {
     std::vector<int> total(b, 0);
     if (!mixed) {
       total = minres_core<double>(rhs.p, xp, b, std::vector<double>(b, tol), maxit);
+      if (verbose) {
+        const int mx = *std::max_element(total.begin(), total.end());
+        std::fprintf(stderr, "[idft] minres fp64: b=%d tol=%.3e max_its=%d\n", b, tol, mx);
+      }
     } else {
       // FP32 inner MINRES on the FP32 cell operator, FP64 iterative refinement:
-      // r = b - (H~ - eps) x in FP64, inner solve to rel. 1e-5, x += delta.
+      // r = b - (H~ - eps) x in FP64, inner solve to rel. inner_tol, x += delta.
       rref.resize(NB(b));
       double* r = rref.p;
       mw32.ensure(NB(b));
@@ -711,12 +772,17 @@ struct idft_ctx {
         std::vector<double> rn = coldot(r, r, b, Djac.p);
         std::vector<double> tl(b);
         bool all = true;
+        double rmax = 0.0;
         for (int j = 0; j < b; ++j) {
           const double nrm = std::sqrt(std::max(rn[j], 0.0));
           if (!std::isfinite(nrm)) throw Error(IDFT_ENAN, "non-finite refinement residual");
+          rmax = std::max(rmax, nrm);
           if (nrm <= tol) tl[j] = INFINITY;  // converged column: start as done
-          else { tl[j] = std::max(tol, 1e-5 * nrm); all = false; }
+          else { tl[j] = std::max(tol, d.inner_tol * nrm); all = false; }
         }
+        if (verbose)
+          std::fprintf(stderr, "[idft] minres mixed: sweep %d used %d max |r|_D %.3e tol %.3e\n",
+                       sweeps, used, rmax, tol);
         if (all) break;
         launch_convert_d2f((int64_t)NB(b), mw32.rhs.p, r, s);
         x32.resize(NB(b));
@@ -755,6 +821,7 @@ struct idft_ctx {
     } else {
       tolCheb = std::min(d.cheb_tol, d.initial_tol_cheb);
     }
+    if (vxcFreq > 0) potEval.assign(pot, pot + t.NV);
     set_potential(pot);
     solve_passes(tolCheb, d.max_cheb_passes, nullptr, nullptr);
     st.cheb_tol_used = tolCheb;
@@ -840,12 +907,19 @@ struct idft_ctx {
     st.t_adjoint_ms = wall_ms() - t0 - tForce;
     st.t_force_ms = tForce;
     lossPrev = loss;
+    // ISF:1250-1254: every vxcFreq-th evaluation writes its input potential
+    if (vxcFreq > 0 && counter % vxcFreq == 0 && (int64_t)potEval.size() == t.NV) {
+      const std::string file = vxcFolder + "/" + vxcPostfix + "_" + std::to_string(counter);
+      if (idft_write_vxc_text(&d, file.c_str(), potEval.data(), 1) != IDFT_OK)
+        throw Error(IDFT_EINVAL, "cannot write v_xc file " + file);
+    }
     counter++;
     st.n_evaluations = counter;
     haveForward = false;
   }
 };
 
+const BfgsCheckpoint& idft::bfgs_checkpoint_of(const idft_ctx* c) { return c->ckpt; }
 
 // ======================================================================= ABI
 namespace {
@@ -875,6 +949,7 @@ idft_ctx* create_ctx(const idft_desc* d, const double* pot_base_q, const double*
     c->B = d->n_eig;
     c->nQ = t.ncell * (int64_t)t.nq;
     c->mixed = d->precision == IDFT_MIXED;
+    c->verbose = std::getenv("IDFT_VERBOSE") != nullptr;
     IDFT_HIP(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
     hipStream_t s = c->s;
     c->dt.Lq.upload(t.Lq.data(), t.Lq.size(), s);
@@ -1260,19 +1335,38 @@ int idft_profile_apply(idft_ctx* c, int32_t ncols, int32_t prec, int32_t reps, d
   });
 }
 
+int idft_set_vxc_output(idft_ctx* c, const char* folder, const char* postfix, int32_t frequency) {
+  if (!c || frequency < 0 || (frequency > 0 && (!folder || !postfix))) return IDFT_EINVAL;
+  c->vxcFreq = frequency;
+  c->vxcFolder = frequency > 0 ? folder : "";
+  c->vxcPostfix = frequency > 0 ? postfix : "";
+  return IDFT_OK;
+}
+
+int idft_bfgs_set_checkpoint(idft_ctx* c, const char* path, int32_t every, int32_t resume) {
+  if (!c || (path && every < 1)) return IDFT_EINVAL;
+  c->ckpt.path = path ? path : "";
+  c->ckpt.every = path ? every : 1;
+  c->ckpt.resume = path && resume != 0;
+  return IDFT_OK;
+}
+
 int idft_set_apply_shape(idft_ctx* c, int32_t prec, int32_t cells_per_wg,
-                         int32_t vectors_per_tile) {
+                         int32_t vectors_per_tile, int32_t gather_rows) {
   if (!c || cells_per_wg < 1 || cells_per_wg > 64 || vectors_per_tile < 16 ||
-      vectors_per_tile > kMaxCols || vectors_per_tile % 16)
+      vectors_per_tile > kMaxCols || vectors_per_tile % 16 || gather_rows < 0 ||
+      gather_rows > 4096)
     return IDFT_EINVAL;
   GemmShape& t = prec == IDFT_MIXED ? c->shape32 : c->shape64;
   t.cells_per_wg = cells_per_wg;
   t.vectors_per_tile = vectors_per_tile;
+  t.gather_rows = gather_rows;
   return IDFT_OK;
 }
 
 int idft_autotune_apply(idft_ctx* c, int32_t ncols, int32_t prec, int32_t reps,
-                        int32_t* cells_per_wg, int32_t* vectors_per_tile, double* ms_best) {
+                        int32_t* cells_per_wg, int32_t* vectors_per_tile, int32_t* gather_rows,
+                        double* ms_best) {
   if (!c || ncols < 1 || ncols > kMaxCols || reps < 1) return IDFT_EINVAL;
   if (!c->haveH) { c->err = "no Hamiltonian"; return IDFT_ESTATE; }
   const bool f32 = prec == IDFT_MIXED;
@@ -1288,19 +1382,29 @@ int idft_autotune_apply(idft_ctx* c, int32_t ncols, int32_t prec, int32_t reps,
       const int e = std::min(vt, vmax);
       if (std::find(widths.begin(), widths.end(), e) == widths.end()) widths.push_back(e);
     }
+    auto trial = [&] {
+      double ms = 0.0;
+      int rc = idft_profile_apply(c, ncols, prec, reps, &ms, nullptr);
+      if (rc) { target = saved; throw Error(rc, c->err); }
+      if (ms < bestMs) { bestMs = ms; best = target; }
+    };
+    target.gather_rows = 0;
     for (int vt : widths) {
       for (int cp : {1, 2, 4, 8}) {
         target.cells_per_wg = cp;
         target.vectors_per_tile = vt;
-        double ms = 0.0;
-        int rc = idft_profile_apply(c, ncols, prec, reps, &ms, nullptr);
-        if (rc) { target = saved; throw Error(rc, c->err); }
-        if (ms < bestMs) { bestMs = ms; best = target; }
+        trial();
       }
+    }
+    target = best;
+    for (int gr : {1, 2, 4, 8}) {
+      target.gather_rows = gr;
+      trial();
     }
     target = best;
     if (cells_per_wg) *cells_per_wg = best.cells_per_wg;
     if (vectors_per_tile) *vectors_per_tile = best.vectors_per_tile;
+    if (gather_rows) *gather_rows = best.gather_rows;
     if (ms_best) *ms_best = bestMs;
   });
 }

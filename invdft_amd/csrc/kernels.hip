@@ -3,10 +3,12 @@
 // Layouts
 //   multivectors  node-major X[node * B + col]
 //   quad data     [cell][q], q = (qz*Q + qy)*Q + qx
-//   H tiles       per cell NEP x NEP (NEP = ne rounded up to 16), stored as
-//                 [rt][kg][lane][s] so that lane l of a wave holds the A operand
-//                 of v_mfma_*_16x16x4 for four consecutive k-steps:
-//                 H[rt*16 + l%16][kg*16 + s*4 + l/16],  s = 0..3.
+//   H tiles       H_e is symmetric, so per cell only the 16x16 tiles (rt, kg)
+//                 with rt <= kg of the NEP x NEP matrix (NEP = ne rounded up
+//                 to 16) are stored, row-major over the upper triangle
+//                 (tile_index), each as [lane][s] so that lane l of a wave
+//                 holds the A operand of v_mfma_*_16x16x4 for four
+//                 consecutive k-steps: H[rt*16 + l%16][kg*16 + s*4 + l/16].
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -41,6 +43,11 @@ inline void check_u32(int64_t n) {
 
 __host__ __device__ constexpr int ne_of(int p) { return (p + 1) * (p + 1) * (p + 1); }
 __host__ __device__ constexpr int nep_of(int p) { return (ne_of(p) + 15) / 16 * 16; }
+// stored tiles per cell and the position of tile (r, k), r <= k, among them
+__host__ __device__ constexpr int ntiles_of(int p) { return (nep_of(p) / 16) * (nep_of(p) / 16 + 1) / 2; }
+__host__ __device__ constexpr int tile_index(int rt_count, int r, int k) {
+  return r * rt_count - r * (r - 1) / 2 + (k - r);
+}
 
 __device__ inline int64_t cell_node(const Geo& g, int64_t cx, int64_t cy, int64_t cz, int l) {
   const int a = l % g.n1, b = (l / g.n1) % g.n1, c = l / (g.n1 * g.n1);
@@ -101,20 +108,44 @@ template <> struct Vec4<double> { typedef double4 type; };
 
 __host__ __device__ constexpr int ldb_of(int bt) { return (bt % 32 == 16) ? bt : bt + 16; }
 
+// A operand of four MFMA k-steps for block row i, k-group k of H_e (Hc = the
+// cell's stored tiles).  k >= i: tile (i, k) as stored, one vector load.
+// k < i: tile (k, i) transposed -- lane l needs T[4s + l/16][l%16], which
+// sits in lane (4s + l/16) + 16 (l%4), slot (l%16)/4 of the stored tile:
+// four scalar loads from the same 16x16 tile.
+template <typename T, int RT>
+__device__ inline typename Vec4<T>::type fetch_tile(const T* __restrict__ Hc, int i, int k,
+                                                     int lane) {
+  typedef typename Vec4<T>::type vec_t;
+  if (k >= i) return reinterpret_cast<const vec_t*>(Hc)[tile_index(RT, i, k) * 64 + lane];
+  const T* t = Hc + tile_index(RT, k, i) * 256 + ((lane & 15) >> 2);
+  const int lb = (lane >> 4) + 16 * (lane & 3);
+  vec_t v;
+  v.x = t[lb * 4];
+  v.y = t[(lb + 4) * 4];
+  v.z = t[(lb + 8) * 4];
+  v.w = t[(lb + 12) * 4];
+  return v;
+}
+
 // Phase 1 of the cell operator.  One work-group handles `cpw` consecutive
 // cells and the column tile [blockIdx.y * 16 NCT, +16 NCT) of the block.  Per
-// cell the tile of (S X)_e is gathered into LDS, the 4 waves share the 16-row
-// tiles of H_e and stream them from HBM with one 16- (f32) or 32-byte (f64)
-// load per lane per four MFMA k-steps, KPF k-groups in flight per wave.
+// cell the tile of (S X)_e is gathered into LDS and each wave computes whole
+// 16-row blocks of H_e (S X)_e, streaming the tiles of its block row from
+// HBM (upper-triangle tiles as stored, lower-triangle ones as transposes of
+// their mirror tiles, which the mirror's own row pass reads too, so the
+// second read is served by the caches), KPF k-groups in flight per wave.
 // Output: Yc[e][l][j] = (H_e (S X)_e)[l][j].  Work-groups are dealt to the 8
 // XCDs in contiguous runs of cells.
 template <typename T, int P, int NCT>
 __global__ __launch_bounds__(kThreads) void k_cell_gemm(Geo g, const T* __restrict__ H,
                                                         const T* __restrict__ S,
                                                         const T* __restrict__ X,
+                                                        const double* __restrict__ xn,
                                                         const double* __restrict__ xcol,
                                                         T* __restrict__ Yc, int B, int cpw) {
   constexpr int NE = ne_of(P), NEP = nep_of(P), RT = NEP / 16, BT = NCT * 16;
+  constexpr int NTILE = ntiles_of(P);
   constexpr int LDB = ldb_of(BT);
   constexpr int KPF = 3;
   __shared__ T xs[NEP * LDB];
@@ -139,21 +170,22 @@ __global__ __launch_bounds__(kThreads) void k_cell_gemm(Geo g, const T* __restri
       if (l < NE && j < bw) {
         const int64_t n = cell_node(g, cx, cy, cz, l);
         v = S[n] * X[n * B + c0 + j];
+        if (xn) v *= (T)xn[n];
         if (xcol) v *= (T)xcol[c0 + j];
       }
       xs[l * LDB + j] = v;
     }
     __syncthreads();
-    const vec_t* Ht = reinterpret_cast<const vec_t*>(H + (size_t)e * NEP * NEP);
+    const T* Hc = H + (size_t)e * NTILE * 256;
     T* yc = Yc + (size_t)e * NE * B + c0;
     for (int rt = wave; rt < RT; rt += 4) {
       acc_t acc[NCT];
 #pragma unroll
       for (int ct = 0; ct < NCT; ++ct) acc[ct] = acc_t{0, 0, 0, 0};
-      const vec_t* row = Ht + (size_t)rt * RT * 64 + lane;
       vec_t buf[KPF];
 #pragma unroll
-      for (int kk = 0; kk < KPF; ++kk) buf[kk] = kk < RT ? row[kk * 64] : vec_t{};
+      for (int kk = 0; kk < KPF; ++kk)
+        buf[kk] = kk < RT ? fetch_tile<T, RT>(Hc, rt, kk, lane) : vec_t{};
 #pragma unroll 1
       for (int kg0 = 0; kg0 < RT; kg0 += KPF) {
 #pragma unroll
@@ -161,7 +193,7 @@ __global__ __launch_bounds__(kThreads) void k_cell_gemm(Geo g, const T* __restri
           const int kg = kg0 + kk;
           if (kg < RT) {
             const vec_t a = buf[kk];
-            if (kg + KPF < RT) buf[kk] = row[(kg + KPF) * 64];
+            if (kg + KPF < RT) buf[kk] = fetch_tile<T, RT>(Hc, rt, kg + KPF, lane);
             const T* xk = xs + (kg * 16 + lk) * LDB + lr;
             const T av[4] = {a.x, a.y, a.z, a.w};
 #pragma unroll
@@ -191,19 +223,21 @@ __global__ __launch_bounds__(kThreads) void k_cell_gemm(Geo g, const T* __restri
 // Phase 2: every node sums its contributions from the (at most 8) cells that
 // share it -- found arithmetically on the structured mesh, so no atomics and
 // no colouring -- and combines them with up to three other blocks:
-//   Y = beta Y + alpha S[n] sum + a ca[j] X + b cb[j] Z + c cc[j] W.
-// Optionally V = xcol[j] Xin (the scaled operator input) and per-column block
-// partials of V.Y (FP64).  Block (bx, by) covers node rows (iy, iz) in
-// [by * R, by * R + R) and x positions [bx * per, bx * per + per) of each row;
-// thread t owns column t % B and x offset t / B for its whole life.
+//   Y = beta Y + alpha S[n] sum + a ca[j] xn[n] X + b cb[j] Z + c cc[j] W
+// (xn, the operator input's node scale, optional).  With `partial`, the
+// per-column block partials of V.Y (FP64) for the scaled operator input
+// V = xcol[j] xn[n] Xin, optionally stored.  Block (bx, by) covers node rows
+// (iy, iz) in [by * R, by * R + R) and x positions [bx * per, bx * per + per)
+// of each row; thread t owns column t % B and x offset t / B for its whole
+// life.
 template <typename T, int P>
 __global__ __launch_bounds__(kThreads) void k_cell_gather(
     Geo g, uint32_t B, uint32_t R, const T* __restrict__ Yc, const T* __restrict__ S,
     T* __restrict__ Y, double alpha, double beta, const T* __restrict__ X, double a,
     const double* __restrict__ ca, const T* __restrict__ Z, double b,
     const double* __restrict__ cb, const T* __restrict__ W, double c,
-    const double* __restrict__ cc, const T* __restrict__ Xin, const double* __restrict__ xcol,
-    T* __restrict__ V, double* __restrict__ partial) {
+    const double* __restrict__ cc, const T* __restrict__ Xin, const double* __restrict__ xn,
+    const double* __restrict__ xcol, T* __restrict__ V, double* __restrict__ partial) {
   constexpr uint32_t n1 = P + 1, ne = n1 * n1 * n1;
   __shared__ double red[kThreads];
   const uint32_t N1 = g.N1, nc = g.nc;
@@ -245,25 +279,37 @@ __global__ __launch_bounds__(kThreads) void k_cell_gather(
           if (q < nc) { czs[nz] = q; azs[nz++] = 0; }
         } else { czs[nz] = q; azs[nz++] = r; }
       }
+      // all (up to 8) contributions are loaded before the first is summed:
+      // predicated, fully unrolled, so the loads are in flight together
+      T c8[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const int kz = k >> 2, ky = (k >> 1) & 1, kx = k & 1;
+        c8[k] = T(0);
+        if (kz < nz && ky < ny && kx < nx) {
+          const uint64_t e = ((uint64_t)czs[kz] * nc + cys[ky]) * nc + cxs[kx];
+          const uint32_t l = (azs[kz] * n1 + ays[ky]) * n1 + axs[kx];
+          c8[k] = Yc[(e * ne + l) * B + j];
+        }
+      }
       double sum = 0.0;
-      for (int kz = 0; kz < nz; ++kz)
-        for (int ky = 0; ky < ny; ++ky)
-          for (int kx = 0; kx < nx; ++kx) {
-            const uint64_t e = ((uint64_t)czs[kz] * nc + cys[ky]) * nc + cxs[kx];
-            const uint32_t l = (azs[kz] * n1 + ays[ky]) * n1 + axs[kx];
-            sum += (double)Yc[(e * ne + l) * B + j];
-          }
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const int kz = k >> 2, ky = (k >> 1) & 1, kx = k & 1;
+        if (kz < nz && ky < ny && kx < nx) sum += (double)c8[k];
+      }
       const uint32_t n = row * N1 + ix;
       const uint32_t i = n * B + j;
+      const double sn = xn ? xn[n] : 1.0;
       double v = alpha * (double)S[n] * sum;
-      if (X) v += caj * (double)X[i];
+      if (X) v += caj * (sn * (double)X[i]);
       if (Z) v += cbj * (double)Z[i];
       if (W) v += ccj * (double)W[i];
       if (beta != 0.0) v += beta * (double)Y[i];
       const T yo = (T)v;
-      if (V) {
-        const T vv = (T)(xs * (double)Xin[i]);
-        V[i] = vv;
+      if (partial) {
+        const T vv = (T)(xs * (sn * (double)Xin[i]));
+        if (V) V[i] = vv;
         dot += (double)vv * (double)yo;
       }
       Y[i] = yo;
@@ -322,11 +368,14 @@ __global__ __launch_bounds__(kThreads) void k_build_h(Geo g, const double* __res
     T2[i] = s;
   }
   __syncthreads();
-  double* o64 = h64 + (size_t)e * NEP * NEP;
-  float* o32 = h32 ? h32 + (size_t)e * NEP * NEP : nullptr;
-  for (int idx = tid; idx < NEP * NEP; idx += kThreads) {
+  constexpr int NTILE = ntiles_of(P);
+  double* o64 = h64 + (size_t)e * NTILE * 256;
+  float* o32 = h32 ? h32 + (size_t)e * NTILE * 256 : nullptr;
+  for (int idx = tid; idx < NTILE * 256; idx += kThreads) {
     const int s = idx % 4, lane = (idx / 4) % 64, blk = idx / 256;
-    const int kg = blk % RT, rt = blk / RT;
+    int rt = 0;
+    while (rt + 1 < RT && tile_index(RT, rt + 1, rt + 1) <= blk) ++rt;
+    const int kg = rt + (blk - tile_index(RT, rt, rt));
     const int row = rt * 16 + lane % 16, col = kg * 16 + s * 4 + lane / 16;
     double v = 0.0;
     if (row < NE && col < NE) {
@@ -875,9 +924,12 @@ __global__ __launch_bounds__(kThreads) void k_minres_alpha(int B, int nblk,
   st[MS_CA2 * B + j] = done ? 0.0 : -s / st[MS_BETA * B + j];
 }
 
-// r_new = y + ca2 r2 (stored in rnew), y <- D r_new, partial dot(r_new, D r_new)
+// r_new = y + ca2 r2 (stored in rnew), partial dot(r_new, D r_new).  The
+// preconditioned vector D r_new is not stored: the next operator
+// application and update scale r_new by D on the fly.
 template <typename T>
-__global__ __launch_bounds__(kThreads) void k_minres_lanczos(int64_t N, int B, T* __restrict__ y,
+__global__ __launch_bounds__(kThreads) void k_minres_lanczos(int64_t N, int B,
+                                                             const T* __restrict__ y,
                                                              const T* __restrict__ r2,
                                                              T* __restrict__ rnew,
                                                              const double* __restrict__ D,
@@ -894,10 +946,8 @@ __global__ __launch_bounds__(kThreads) void k_minres_lanczos(int64_t N, int B, T
       const int64_t i = n * B + col;
       const double rn = (double)y[i] + c * (double)r2[i];
       const T rt = (T)rn;
-      const double z = D[n] * (double)rt;
       rnew[i] = rt;
-      y[i] = (T)z;
-      s += (double)rt * z;
+      s += (double)rt * (D[n] * (double)rt);
     }
   }
   red[tid] = s;
@@ -944,7 +994,7 @@ __global__ __launch_bounds__(kThreads) void k_minres_givens(int B, int nblk,
   S[MS_CS * B + j] = ncs;
   S[MS_SN * B + j] = nsn;
   S[MS_PHIBAR * B + j] = phibar;
-  S[MS_CWV * B + j] = 1.0 / gamma;
+  S[MS_CWV * B + j] = S[MS_CV * B + j] / gamma;  // v = cv D r: cv of this step
   S[MS_CWW1 * B + j] = -oldeps / gamma;
   S[MS_CWW * B + j] = -delta / gamma;
   S[MS_CX * B + j] = phi;
@@ -956,30 +1006,19 @@ __global__ __launch_bounds__(kThreads) void k_minres_givens(int B, int nblk,
   S[MS_CR1 * B + j] = done ? 0.0 : -beta / oldb;
 }
 
-// w_new = cwv v + cww1 w1 + cww w  (written over w1);  x += cx w_new
+// w_new = cwv D r + cww1 w1 + cww w  (written over w1);  x += cx w_new,
+// where cwv D r = v / gamma, v = cv D r the step's Lanczos vector.
 template <typename T>
-__global__ void k_minres_update(uint32_t NB, uint32_t B, const T* __restrict__ v,
-                                T* __restrict__ w1, const T* __restrict__ w,
-                                T* __restrict__ x, const double* __restrict__ st) {
+__global__ void k_minres_update(uint32_t NB, uint32_t B, const T* __restrict__ r,
+                                const double* __restrict__ D, T* __restrict__ w1,
+                                const T* __restrict__ w, T* __restrict__ x,
+                                const double* __restrict__ st) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < NB; i += gridDim.x * blockDim.x) {
-    const uint32_t j = i % B;
-    const double wn = st[MS_CWV * B + j] * (double)v[i] + st[MS_CWW1 * B + j] * (double)w1[i] +
-                      st[MS_CWW * B + j] * (double)w[i];
+    const uint32_t n = i / B, j = i - n * B;
+    const double wn = st[MS_CWV * B + j] * (D[n] * (double)r[i]) +
+                      st[MS_CWW1 * B + j] * (double)w1[i] + st[MS_CWW * B + j] * (double)w[i];
     w1[i] = (T)wn;
     x[i] = (T)((double)x[i] + st[MS_CX * B + j] * wn);
-  }
-}
-
-// v = cv y;  y = -eps v + cr1 r1  (the operator application then adds H~ v)
-template <typename T>
-__global__ void k_minres_prep(uint32_t NB, uint32_t B, T* __restrict__ y, T* __restrict__ v,
-                              const T* __restrict__ r1, const double* __restrict__ negEig,
-                              const double* __restrict__ st) {
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < NB; i += gridDim.x * blockDim.x) {
-    const uint32_t j = i % B;
-    const T vv = (T)(st[MS_CV * B + j] * (double)y[i]);
-    v[i] = vv;
-    y[i] = (T)(negEig[j] * (double)vv + st[MS_CR1 * B + j] * (double)r1[i]);
   }
 }
 
@@ -1041,7 +1080,7 @@ inline int grid_for(int64_t n) {
 }  // namespace
 
 // ============================================================== launchers
-int64_t hamiltonian_tile_elems(int p) { return (int64_t)nep_of(p) * nep_of(p); }
+int64_t hamiltonian_tile_elems(int p) { return (int64_t)ntiles_of(p) * 256; }
 
 void launch_build_hamiltonian(const Geo& g, const DevTables& t, double hh, const double* veff,
                               const double* jxw, double* h64, float* h32, hipStream_t s) {
@@ -1057,8 +1096,8 @@ void launch_build_hamiltonian(const Geo& g, const DevTables& t, double hh, const
 }
 
 template <typename T, int P>
-static void cell_gemm_p(const Geo& g, const T* H, const T* S, const T* X, const double* xcol,
-                        T* Yc, int B, const GemmShape& shape, hipStream_t s) {
+static void cell_gemm_p(const Geo& g, const T* H, const T* S, const T* X, const double* xn,
+                        const double* xcol, T* Yc, int B, const GemmShape& shape, hipStream_t s) {
   const int vt = std::min(shape.vectors_per_tile, ((B + 15) / 16) * 16);
   const int nct = std::max(1, std::min(5, vt / 16));
   const int cpw = std::max(1, shape.cells_per_wg);
@@ -1066,32 +1105,34 @@ static void cell_gemm_p(const Geo& g, const T* H, const T* S, const T* X, const 
   dim3 grid((unsigned)(8 * ((nwg + 7) / 8)), (unsigned)((B + 16 * nct - 1) / (16 * nct))),
       block(kThreads);
   switch (nct) {
-    case 1: k_cell_gemm<T, P, 1><<<grid, block, 0, s>>>(g, H, S, X, xcol, Yc, B, cpw); break;
-    case 2: k_cell_gemm<T, P, 2><<<grid, block, 0, s>>>(g, H, S, X, xcol, Yc, B, cpw); break;
-    case 3: k_cell_gemm<T, P, 3><<<grid, block, 0, s>>>(g, H, S, X, xcol, Yc, B, cpw); break;
-    case 4: k_cell_gemm<T, P, 4><<<grid, block, 0, s>>>(g, H, S, X, xcol, Yc, B, cpw); break;
-    case 5: k_cell_gemm<T, P, 5><<<grid, block, 0, s>>>(g, H, S, X, xcol, Yc, B, cpw); break;
+    case 1: k_cell_gemm<T, P, 1><<<grid, block, 0, s>>>(g, H, S, X, xn, xcol, Yc, B, cpw); break;
+    case 2: k_cell_gemm<T, P, 2><<<grid, block, 0, s>>>(g, H, S, X, xn, xcol, Yc, B, cpw); break;
+    case 3: k_cell_gemm<T, P, 3><<<grid, block, 0, s>>>(g, H, S, X, xn, xcol, Yc, B, cpw); break;
+    case 4: k_cell_gemm<T, P, 4><<<grid, block, 0, s>>>(g, H, S, X, xn, xcol, Yc, B, cpw); break;
+    case 5: k_cell_gemm<T, P, 5><<<grid, block, 0, s>>>(g, H, S, X, xn, xcol, Yc, B, cpw); break;
   }
 }
 
 int64_t cell_scratch_elems(const Geo& g, int B) { return g.ncell * (int64_t)ne_of(g.p) * B; }
 
 // Gather grid: x chunks of kThreads / B nodes times groups of R node rows,
-// with R chosen so that the grid has at most about kGatherBlocks blocks.
-static void gather_grid(const Geo& g, int B, unsigned& gx, unsigned& gy, uint32_t& R) {
+// with R = `rows_req` if positive, else chosen so that the grid has at most
+// about kGatherBlocks blocks.
+static void gather_grid(const Geo& g, int B, int rows_req, unsigned& gx, unsigned& gy,
+                        uint32_t& R) {
   const int per = kThreads / B;
   gx = (unsigned)((g.N1 + per - 1) / per);
   const int64_t rows = (int64_t)g.N1 * g.N1;
-  int64_t r = (rows * gx + kGatherBlocks - 1) / kGatherBlocks;
+  int64_t r = rows_req > 0 ? rows_req : (rows * gx + kGatherBlocks - 1) / kGatherBlocks;
   if (r < 1) r = 1;
   R = (uint32_t)r;
   gy = (unsigned)((rows + r - 1) / r);
 }
 
-int gather_blocks(const Geo& g, int B) {
+int gather_blocks(const Geo& g, int B, int rows) {
   unsigned gx, gy;
   uint32_t R;
-  gather_grid(g, B, gx, gy, R);
+  gather_grid(g, B, rows, gx, gy, R);
   return (int)(gx * gy);
 }
 
@@ -1100,10 +1141,10 @@ void launch_cell_gemm(const Geo& g, const T* H, const T* S, const T* X, const do
                       int B, const GemmShape& shape, hipStream_t s) {
   if (B < 1 || B > kMaxCols) throw Error(IDFT_EINVAL, "block width out of range");
   switch (g.p) {
-    case 2: cell_gemm_p<T, 2>(g, H, S, X, xcol, Yc, B, shape, s); break;
-    case 3: cell_gemm_p<T, 3>(g, H, S, X, xcol, Yc, B, shape, s); break;
-    case 4: cell_gemm_p<T, 4>(g, H, S, X, xcol, Yc, B, shape, s); break;
-    case 5: cell_gemm_p<T, 5>(g, H, S, X, xcol, Yc, B, shape, s); break;
+    case 2: cell_gemm_p<T, 2>(g, H, S, X, nullptr, xcol, Yc, B, shape, s); break;
+    case 3: cell_gemm_p<T, 3>(g, H, S, X, nullptr, xcol, Yc, B, shape, s); break;
+    case 4: cell_gemm_p<T, 4>(g, H, S, X, nullptr, xcol, Yc, B, shape, s); break;
+    case 5: cell_gemm_p<T, 5>(g, H, S, X, nullptr, xcol, Yc, B, shape, s); break;
     default: throw Error(IDFT_EINVAL, "degree not supported by the HIP kernels (2..5)");
   }
   IDFT_HIP(hipGetLastError());
@@ -1118,22 +1159,22 @@ int launch_cell_apply(const Geo& g, const T* H, const T* S, const T* X, T* Yc, i
                       const ApplyEpilogue<T>& ep, const GemmShape& shape, hipStream_t s) {
   if (B < 1 || B > kMaxCols) throw Error(IDFT_EINVAL, "block width out of range");
   switch (g.p) {
-    case 2: cell_gemm_p<T, 2>(g, H, S, X, ep.xcol, Yc, B, shape, s); break;
-    case 3: cell_gemm_p<T, 3>(g, H, S, X, ep.xcol, Yc, B, shape, s); break;
-    case 4: cell_gemm_p<T, 4>(g, H, S, X, ep.xcol, Yc, B, shape, s); break;
-    case 5: cell_gemm_p<T, 5>(g, H, S, X, ep.xcol, Yc, B, shape, s); break;
+    case 2: cell_gemm_p<T, 2>(g, H, S, X, ep.xn, ep.xcol, Yc, B, shape, s); break;
+    case 3: cell_gemm_p<T, 3>(g, H, S, X, ep.xn, ep.xcol, Yc, B, shape, s); break;
+    case 4: cell_gemm_p<T, 4>(g, H, S, X, ep.xn, ep.xcol, Yc, B, shape, s); break;
+    case 5: cell_gemm_p<T, 5>(g, H, S, X, ep.xn, ep.xcol, Yc, B, shape, s); break;
     default: throw Error(IDFT_EINVAL, "degree not supported by the HIP kernels (2..5)");
   }
   IDFT_HIP(hipGetLastError());
   check_u32(g.N * B);
   unsigned gx, gy;
   uint32_t R;
-  gather_grid(g, B, gx, gy, R);
+  gather_grid(g, B, shape.gather_rows, gx, gy, R);
   dim3 grid(gx, gy);
 #define IDFT_GATHER(PP)                                                                       \
   k_cell_gather<T, PP><<<grid, kThreads, 0, s>>>(g, (uint32_t)B, R, Yc, S, ep.Y, ep.alpha,   \
                                                  ep.beta, ep.X, ep.a, ep.ca, ep.Z, ep.b, ep.cb, \
-                                                 ep.W, ep.c, ep.cc, X, ep.xcol, ep.V,         \
+                                                 ep.W, ep.c, ep.cc, X, ep.xn, ep.xcol, ep.V,  \
                                                  ep.dot_partial)
   switch (g.p) {
     case 2: IDFT_GATHER(2); break;
@@ -1360,16 +1401,16 @@ void launch_minres_alpha(int B, int nblk, const double* partial, double* st, hip
   IDFT_HIP(hipGetLastError());
 }
 template <typename T>
-int launch_minres_lanczos(int64_t N, int B, T* y, const T* r2, T* rnew, const double* D,
+int launch_minres_lanczos(int64_t N, int B, const T* y, const T* r2, T* rnew, const double* D,
                           const double* st, double* partial, hipStream_t s) {
   const int nb = coldot_blocks(N, B);
   k_minres_lanczos<T><<<nb, kThreads, 0, s>>>(N, B, y, r2, rnew, D, st, partial);
   IDFT_HIP(hipGetLastError());
   return nb;
 }
-template int launch_minres_lanczos<float>(int64_t, int, float*, const float*, float*,
+template int launch_minres_lanczos<float>(int64_t, int, const float*, const float*, float*,
                                           const double*, const double*, double*, hipStream_t);
-template int launch_minres_lanczos<double>(int64_t, int, double*, const double*, double*,
+template int launch_minres_lanczos<double>(int64_t, int, const double*, const double*, double*,
                                            const double*, const double*, double*, hipStream_t);
 void launch_minres_givens(int B, int nblk, const double* partial, const double* negEig, double* st,
                           hipStream_t s) {
@@ -1377,29 +1418,17 @@ void launch_minres_givens(int B, int nblk, const double* partial, const double* 
   IDFT_HIP(hipGetLastError());
 }
 template <typename T>
-void launch_minres_update(int64_t N, int B, const T* v, T* w1, const T* w, T* x,
+void launch_minres_update(int64_t N, int B, const T* r, const double* D, T* w1, const T* w, T* x,
                           const double* st, hipStream_t s) {
   check_u32(N * B);
-  k_minres_update<T><<<grid_for(N * B), kThreads, 0, s>>>((uint32_t)(N * B), (uint32_t)B, v, w1, w,
-                                                          x, st);
+  k_minres_update<T><<<grid_for(N * B), kThreads, 0, s>>>((uint32_t)(N * B), (uint32_t)B, r, D,
+                                                          w1, w, x, st);
   IDFT_HIP(hipGetLastError());
 }
-template <typename T>
-void launch_minres_prep(int64_t N, int B, T* y, T* v, const T* r1, const double* negEig,
-                        const double* st, hipStream_t s) {
-  check_u32(N * B);
-  k_minres_prep<T><<<grid_for(N * B), kThreads, 0, s>>>((uint32_t)(N * B), (uint32_t)B, y, v, r1,
-                                                        negEig, st);
-  IDFT_HIP(hipGetLastError());
-}
-template void launch_minres_prep<float>(int64_t, int, float*, float*, const float*, const double*,
-                                        const double*, hipStream_t);
-template void launch_minres_prep<double>(int64_t, int, double*, double*, const double*,
-                                         const double*, const double*, hipStream_t);
-template void launch_minres_update<float>(int64_t, int, const float*, float*, const float*,
-                                          float*, const double*, hipStream_t);
-template void launch_minres_update<double>(int64_t, int, const double*, double*, const double*,
-                                           double*, const double*, hipStream_t);
+template void launch_minres_update<float>(int64_t, int, const float*, const double*, float*,
+                                          const float*, float*, const double*, hipStream_t);
+template void launch_minres_update<double>(int64_t, int, const double*, const double*, double*,
+                                           const double*, double*, const double*, hipStream_t);
 void launch_copy_done(int B, const double* st, int* flags, hipStream_t s) {
   k_copy_done<<<(B + 63) / 64, 64, 0, s>>>(B, st, flags);
   IDFT_HIP(hipGetLastError());

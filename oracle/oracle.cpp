@@ -1212,6 +1212,8 @@ void idft_desc_default(idft_desc* d) {
   d->frac_occ_tol = 1e-8; d->additional_states = 0; d->block_size = 0;
   d->rho_tol_constraints = 1e-6; d->gradient_mode = IDFT_GRADIENT_REFERENCE;
   d->lanczos_steps = 12; d->refine_every = 40; d->check_every = 8; d->seed = 1;
+  d->minres_graph = 1;  // HIP launch option; no effect here
+  d->inner_tol = 1e-5;  // mixed-precision option; no effect here
 }
 
 static bool desc_ok(const idft_desc* d) {

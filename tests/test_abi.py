@@ -26,7 +26,7 @@ def hip_host():
 
 
 def test_hip_library_exports_every_declared_symbol(hip_host):
-    for name in _declared("idft.h") + _declared("idft_hip.h"):
+    for name in _declared("idft.h") + _declared("idft_hip.h") + _declared("idft_io.h"):
         assert hasattr(hip_host, name), name
 
 

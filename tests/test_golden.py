@@ -12,17 +12,20 @@ FIX = os.path.join(HERE, "golden", "small_problem.npz")
 TIGHT = dict(cheb_tol=1e-12, initial_tol_cheb=1e-12)
 
 
-def _problem():
+def _problem(lib):
     z = np.load(FIX)  # plain arrays, no pickle
-    d = abi.Desc()
-    ctypes.memmove(ctypes.byref(d), z["desc"].tobytes(), ctypes.sizeof(d))
+    raw = z["desc"].tobytes()
+    # the fixture holds the descriptor bytes it was written with; fields added
+    # at the end of the struct since then keep their defaults
+    d = abi.default_desc(lib)
+    ctypes.memmove(ctypes.byref(d), raw, min(len(raw), ctypes.sizeof(d)))
     prob = synthetic.Problem(d, z["pot_base_q"], z["rho_target_q"], z["weight_q"],
                              z["rho_target_nodal"], z["vxc0"])
     return prob, z
 
 
 def _check(lib, dens_tol, force_tol):
-    prob, z = _problem()
+    prob, z = _problem(lib)
     ctx = prob.context(lib, **TIGHT)
     ctx.set_potential(prob.vxc0)
     ev, _, _ = ctx.eigensolve(1e-12, 200)

@@ -125,6 +125,42 @@ def test_adjoint_contract(glib, olib):
     assert np.all(np.abs(G["its"] - O["its"]) <= 3)
 
 
+@pytest.mark.parametrize("prec", [abi.FP64, abi.MIXED])
+def test_minres_graph_replay_matches_launches(glib, olib, prec):
+    """MINRES steps replayed from a captured HIP graph (check_every steps per
+    replay, host convergence checks between replays) give bit-identical
+    solutions and iteration counts to the same steps launched one by one,
+    over two solves in a row on each context."""
+    _say("minres_graph", prec)
+    prob = small_problem(olib)
+    o = prob.context(olib)
+    o.set_potential(prob.vxc0)
+    ev, _, _ = o.eigensolve(1e-11, 60)
+    occ, fermi = o.occupancies()
+    psi = o.eigenvectors()
+    u = prob.rho_target_q - 0.5 * o.density()
+    B = psi.shape[1]
+    pairs = [v for i in range(B) for j in range(B) if abs(ev[i] - ev[j]) < 0.002 for v in (i, j)]
+    out = {}
+    for graph in (0, 1):
+        g = prob.context(glib, precision=prec, minres_graph=graph)
+        g.set_potential(prob.vxc0)
+        mpsi, _ = _adjoint_setup(g, prob, psi, occ, ev.copy(), u, pairs, fermi)
+        runs = []
+        for _ in range(2):
+            x = g.mv(B)
+            g.adj_compute_rhs(None, x, g.mv(B))
+            sol = g.mv(B)
+            its = g.minres_solve(sol, None, 1e-12, 3000, distribute=False)
+            runs.append((sol.get(), np.array(its)))
+        out[graph] = runs
+        del mpsi
+    assert np.max(out[0][0][1]) > 16  # several replays of the 8-step graph
+    for k in range(2):
+        assert np.array_equal(out[0][k][1], out[1][k][1])
+        assert np.array_equal(out[0][k][0], out[1][k][0])
+
+
 @pytest.mark.parametrize("mode", [abi.GRADIENT_REFERENCE, abi.GRADIENT_EXACT])
 def test_get_force(glib, olib, mode):
     _say("get_force", mode)
@@ -176,6 +212,11 @@ def test_every_tile_shape_keeps_parity(glib, olib, p, ncols):
             for prec, tol in ((abi.FP64, 1e-13), (abi.MIXED, 2e-6)):
                 g.set_apply_shape(prec, cp, vt)
                 assert rel(g.apply_hamiltonian(x, precision=prec), yo) < tol, (cp, vt, prec)
+    for gr in (1, 2, 3, 7, 4096):
+        for prec, tol in ((abi.FP64, 1e-13), (abi.MIXED, 2e-6)):
+            g.set_apply_shape(prec, 1, 16, gr)
+            assert rel(g.apply_hamiltonian(x, precision=prec), yo) < tol, (gr, prec)
     best = g.autotune_apply(ncols, abi.FP64, reps=2)
     assert best["cells_per_wg"] in (1, 2, 4, 8) and best["vectors_per_tile"] % 16 == 0
+    assert best["gather_rows"] in (0, 1, 2, 4, 8)
     assert rel(g.apply_hamiltonian(x), yo) < 1e-13
