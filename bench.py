@@ -133,6 +133,35 @@ def kernels_only(lib, cfg, shape, reps):
           flush=True)
 
 
+def uneven_batch(lib, count, steps, workers):
+    """Uneven batch of independent small inverse problems: problems/s and
+    per-problem p50/p99 seconds for `steps` L-BFGS iterations each, mixed
+    precision on `workers` concurrent contexts against sequential FP64
+    solves on the same GPU."""
+    from invdft_amd import abi, batch
+    params = batch.draw_problems(count)
+    t0 = time.perf_counter()
+    probs = []
+    for k in range(0, count, 8):
+        probs += batch.build_problems(lib, params[k:k + 8])
+        log(f"  set-up {len(probs)}/{count} problems ({time.perf_counter() - t0:.1f} s)")
+    tol_kw = dict(cheb_tol=1e-8, initial_tol_cheb=1e-8)
+    res_m, wall_m = batch.run_concurrent(lib, probs, abi.MIXED, steps, workers, tol_kw)
+    log(f"  mixed, {workers} concurrent contexts: {wall_m:.1f} s")
+    res_f, wall_f = batch.run_sequential(lib, probs, abi.FP64, steps, tol_kw)
+    log(f"  FP64 sequential: {wall_f:.1f} s")
+    sm, sf = batch.summary(res_m, wall_m), batch.summary(res_f, wall_f)
+    out = {"metric": f"problems/s, uneven batch of {count} inverse problems x {steps} BFGS steps",
+           "value": sm["problems_per_s"], "unit": "problems/s", "n_gpus": 1,
+           "higher_is_better": True, "dtype": "f32-mfma+f64", "data": "synthetic",
+           "config": {"workload": "uneven_batch", "problems": count, "bfgs_steps": steps,
+                      "cells": "U{8..20}^3", "degree": "U{3,4,5}", "orbitals": "U{1..16}",
+                      "workers": workers},
+           "mixed_concurrent": sm, "fp64_sequential": sf,
+           "speedup": round(sf["wall_s"] / sm["wall_s"], 3)}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -147,6 +176,10 @@ def main():
     ap.add_argument("--shape", type=int, nargs=3, default=[1, 16, 0],
                     metavar=("CELLS_PER_WG", "VECTORS_PER_TILE", "GATHER_ROWS"))
     ap.add_argument("--reps", type=int, default=5)
+    ap.add_argument("--batch-count", type=int, default=96,
+                    help="problems of --config uneven_batch")
+    ap.add_argument("--batch-steps", type=int, default=10)
+    ap.add_argument("--workers", type=int, default=4)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (--gpus {args.gpus} requested); the multi-GPU "
@@ -157,6 +190,9 @@ def main():
     from invdft_amd import abi, synthetic
 
     lib = invdft_amd.load()
+    if args.config == "uneven_batch":
+        uneven_batch(lib, args.batch_count, args.batch_steps, args.workers)
+        return
     cfg = dict(synthetic.CONFIGS[args.config])
     if args.kernels_only:
         kernels_only(lib, cfg, args.shape, args.reps)

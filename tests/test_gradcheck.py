@@ -36,18 +36,25 @@ def test_autograd_wrapper_matches_get_force(olib):
 
 @pytest.mark.gpu
 def test_gradient_vs_central_differences_12c_p4(glib):
+    """The adjoint low-density mask (p zeroed where rho_target < rho_tol,
+    InverseDFTEngine.cpp:1547-1573) is part of the reference's gradient but
+    not of the loss's derivative, so the exact check runs with it off
+    (rho_tol 0); with the reference's rho_tol = 1e-6 the gradient differs
+    from the loss's derivative by the masked far-field nodes' share (the
+    oracle shows 3e-5 of max |g.d| on this mesh, independent of h)."""
     print("CASE gradcheck_12c_p4 32 directions", flush=True)
     cfg = dict(synthetic.CONFIGS["gradcheck_12c_p4"])
     prob = synthetic.make_problem(glib, seed=4, target_tol=1e-11, **cfg)
     tight = dict(cheb_tol=1e-11, initial_tol_cheb=1e-11, gradient_mode=abi.GRADIENT_EXACT)
-    ctx = prob.context(glib, **tight)
+    ctx = prob.context(glib, rho_tol_constraints=0.0, **tight)
     v0 = prob.vxc0.copy()
     v = torch.tensor(v0, requires_grad=True)
     InverseDFTLoss.apply(v, ctx).backward()
     g = v.grad.numpy()
+    g_masked, _ = prob.context(glib, **tight).get_force(v0)
     rng = np.random.default_rng(4)
     h = 1e-4
-    fd, an = [], []
+    fd, an, am = [], [], []
     for _ in range(32):
         d = rng.standard_normal(v0.size)
         d /= np.linalg.norm(d)
@@ -55,7 +62,10 @@ def test_gradient_vs_central_differences_12c_p4(glib):
         lm = ctx.forward(v0 - h * d)
         fd.append((lp - lm) / (2 * h))
         an.append(float(g @ d))
-    fd, an = np.array(fd), np.array(an)
+        am.append(float(g_masked @ d))
+    fd, an, am = np.array(fd), np.array(an), np.array(am)
     err = np.max(np.abs(fd - an)) / np.max(np.abs(an))
-    print(f"max |fd - g.d| / max |g.d| = {err:.3e}", flush=True)
+    err_m = np.max(np.abs(fd - am)) / np.max(np.abs(an))
+    print(f"max |fd - g.d| / max |g.d| = {err:.3e} (masked gradient {err_m:.3e})", flush=True)
     assert err < 1e-5
+    assert err_m < 2e-4
