@@ -93,8 +93,10 @@ typedef struct idft_desc {
   int32_t minres_graph;       /* HIP: replay check_every MINRES steps as one
                                  captured HIP graph (1) or launch each kernel
                                  (0); same results either way                  */
-  double inner_tol;           /* MIXED: FP32 inner MINRES stops at this factor
-                                 times the FP64 refinement residual            */
+  double relax_ratio;         /* MIXED: adjoint MINRES switches from the FP64 to
+                                 the FP32 cell operator once every open
+                                 column's residual estimate is below
+                                 relax_ratio times the tolerance (0: never)    */
 } idft_desc;
 
 typedef struct idft_sizes {

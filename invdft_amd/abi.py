@@ -30,7 +30,7 @@ class Desc(C.Structure):
         ("additional_states", C.c_int32), ("block_size", C.c_int32),
         ("rho_tol_constraints", C.c_double), ("gradient_mode", C.c_int32),
         ("lanczos_steps", C.c_int32), ("refine_every", C.c_int32), ("check_every", C.c_int32),
-        ("seed", C.c_uint64), ("minres_graph", C.c_int32), ("inner_tol", C.c_double),
+        ("seed", C.c_uint64), ("minres_graph", C.c_int32), ("relax_ratio", C.c_double),
     ]
 
 

@@ -147,9 +147,12 @@ struct GemmShape {
   int vectors_per_tile = 80;  // columns per launch tile (rounded to 16; >= B: one pass)
   int gather_rows = 0;        // node rows per gather work-group (0: about kGatherBlocks groups)
 };
-template <typename T>
-int launch_cell_apply(const Geo& g, const T* H, const T* S, const T* Xin, T* Yc, int B,
-                      const ApplyEpilogue<T>& ep, const GemmShape& shape, hipStream_t s);
+// T: cell matrices, cell scratch Yc and MFMA arithmetic; TV: the node-major
+// vectors (Xin and the epilogue's blocks).  Sg (type T) scales the input of
+// the cell products, S (type TV) the gathered sums.
+template <typename T, typename TV>
+int launch_cell_apply(const Geo& g, const T* H, const T* Sg, const TV* S, const TV* Xin, T* Yc,
+                      int B, const ApplyEpilogue<TV>& ep, const GemmShape& shape, hipStream_t s);
 int64_t cell_scratch_elems(const Geo& g, int B);
 // Phase 1 alone (per-cell products into Yc), for profiling.
 template <typename T>

@@ -76,8 +76,8 @@ struct idft_ctx {
   GemmShape shape64, shape32;  // cell-product tile shapes (autotuned)
   bool haveH = false;
   // eigen state (Loewdin vectors) and work
-  DevBuf<double> X, X2, HX, Xtmp, T2buf, rref, psiAll, force;
-  DevBuf<float> F0, F1, F2, Rf, x32;
+  DevBuf<double> X, X2, HX, Xtmp, T2buf, rref, xdelta, psiAll, force;
+  DevBuf<float> F0, F1, F2, Rf;
   DevBuf<double> ydev;
   bool haveHX = false;  // HX == H~ X for the current X and Hamiltonian
   DevBuf<double> partial, small, small2, occD;
@@ -97,7 +97,6 @@ struct idft_ctx {
   idft_stats st{};
   AdjointState adj;
   MinresWork<double> mw64;
-  MinresWork<float> mw32;
   DevBuf<double> mst, mx;
   DevBuf<int> mflags;
   idft_mv mvPsi, mvX;
@@ -115,14 +114,24 @@ struct idft_ctx {
       Yc64.resize(cell_scratch_elems(g, b));
       st.applies_fp64++;
       st.apply_cols_fp64 += b;
-      return launch_cell_apply<double>(g, H64.p, S.p, Xin, Yc64.p, b, ep, shape64, s);
+      return launch_cell_apply<double, double>(g, H64.p, S.p, S.p, Xin, Yc64.p, b, ep, shape64, s);
     } else {
       if (!H32.p) throw Error(IDFT_ESTATE, "FP32 Hamiltonian not built (precision is FP64)");
       Yc32.resize(cell_scratch_elems(g, b));
       st.applies_fp32++;
       st.apply_cols_fp32 += b;
-      return launch_cell_apply<float>(g, H32.p, S32.p, Xin, Yc32.p, b, ep, shape32, s);
+      return launch_cell_apply<float, float>(g, H32.p, S32.p, S32.p, Xin, Yc32.p, b, ep, shape32,
+                                             s);
     }
+  }
+  // FP32 cell operator (FP32 H_e, MFMA and cell scratch) on FP64 vectors:
+  // input scaled and rounded once, gathered sums and epilogue in FP64.
+  int apply_f32op(const double* Xin, int b, const ApplyEpilogue<double>& ep) {
+    if (!H32.p) throw Error(IDFT_ESTATE, "FP32 Hamiltonian not built (precision is FP64)");
+    Yc32.resize(cell_scratch_elems(g, b));
+    st.applies_fp32++;
+    st.apply_cols_fp32 += b;
+    return launch_cell_apply<float, double>(g, H32.p, S32.p, S.p, Xin, Yc32.p, b, ep, shape32, s);
   }
   // Y = alpha S H S X + beta Y
   template <typename T>
@@ -619,18 +628,15 @@ struct idft_ctx {
   }
 
   // ------------------------------------------------------------ MINRES
-  template <typename T>
-  MinresWork<T>& work() {
-    if constexpr (std::is_same<T, double>::value) return mw64;
-    else return mw32;
-  }
-
-  // Preconditioned MINRES from x = 0 (x accumulates), per-column tolerance.
-  // x (same precision as the recurrence) accumulates the solution increment.
-  template <typename T>
-  std::vector<int> minres_core(const T* rhs, T* x, int b, const std::vector<double>& tol,
-                               int maxit) {
-    MinresWork<T>& W = work<T>();
+  // Preconditioned MINRES (FP64 recurrence) from x = 0 (x accumulates), per-
+  // column tolerance.  relax > 0 (MIXED): the FP64 cell operator is applied
+  // until every open column's residual estimate phibar is below relax times
+  // its tolerance, the FP32 cell operator after that -- the matvec accuracy
+  // a Krylov method needs falls as its residual falls (inexact-Krylov
+  // relaxation), so the late steps tolerate FP32 products.
+  std::vector<int> minres_core(const double* rhs, double* x, int b, const std::vector<double>& tol,
+                               int maxit, double relax) {
+    MinresWork<double>& W = mw64;
     W.ensure(NB(b));
     mst.resize((size_t)MS_COUNT * b);
     mflags.resize(b);
@@ -638,19 +644,23 @@ struct idft_ctx {
     IDFT_HIP(hipMemcpyAsync(mst.p + (size_t)MS_TOL * b, mst.p, b * sizeof(double),
                             hipMemcpyDeviceToDevice, s));
     const int64_t N = t.N;
-    launch_lincomb<T>(N, b, W.r1.p, 0.0, rhs, 1.0, nullptr, nullptr, 0.0, nullptr, nullptr, s);
-    launch_lincomb<T>(N, b, W.r2.p, 0.0, rhs, 1.0, nullptr, nullptr, 0.0, nullptr, nullptr, s);
-    launch_lincomb<T>(N, b, W.w1.p, 0.0, nullptr, 0.0, nullptr, nullptr, 0.0, nullptr, nullptr, s);
-    launch_lincomb<T>(N, b, W.w.p, 0.0, nullptr, 0.0, nullptr, nullptr, 0.0, nullptr, nullptr, s);
-    const GemmShape& shp = std::is_same<T, double>::value ? shape64 : shape32;
-    partial.resize((size_t)std::max(1024, gather_blocks(g, b, shp.gather_rows)) * b);
-    int nb = launch_coldot_partial<T>(N, b, W.r1.p, W.r1.p, Djac.p, partial.p, s);
+    launch_lincomb<double>(N, b, W.r1.p, 0.0, rhs, 1.0, nullptr, nullptr, 0.0, nullptr, nullptr, s);
+    launch_lincomb<double>(N, b, W.r2.p, 0.0, rhs, 1.0, nullptr, nullptr, 0.0, nullptr, nullptr, s);
+    launch_lincomb<double>(N, b, W.w1.p, 0.0, nullptr, 0.0, nullptr, nullptr, 0.0, nullptr, nullptr, s);
+    launch_lincomb<double>(N, b, W.w.p, 0.0, nullptr, 0.0, nullptr, nullptr, 0.0, nullptr, nullptr, s);
+    const bool canRelax = relax > 0.0 && H32.p != nullptr;
+    int nblk = std::max(1024, gather_blocks(g, b, shape64.gather_rows));
+    if (canRelax) nblk = std::max(nblk, gather_blocks(g, b, shape32.gather_rows));
+    partial.resize((size_t)nblk * b);
+    int nb = launch_coldot_partial<double>(N, b, W.r1.p, W.r1.p, Djac.p, partial.p, s);
     launch_minres_start(b, nb, partial.p, adj.dnegEig.p, mst.p, s);
     const int check = std::max(1, d.check_every);
     std::vector<int> flags(b);
+    std::vector<double> phibar(b);
+    bool f32op = false;
     // r2 = r_k (newest Lanczos residual), r1 = r_{k-1}; the step's Lanczos
     // vector v = cv D r2 is formed on the fly (not stored)
-    T *r1 = W.r1.p, *r2 = W.r2.p, *w1 = W.w1.p, *w = W.w.p;
+    double *r1 = W.r1.p, *r2 = W.r2.p, *w1 = W.w1.p, *w = W.w.p;
     // one MINRES step: 5 launches, all arguments device-resident (scalars in
     // mst, done flags per column), so a run of steps can be replayed as a
     // graph.  The two buffer swaps return to the start after two steps.
@@ -658,41 +668,58 @@ struct idft_ctx {
     // (update).
     auto step = [&] {
       // y <- H~ v - eps v + cr1 r1, v = cv D r2;  partial dots of v.y  (one apply)
-      ApplyEpilogue<T> ep;
+      ApplyEpilogue<double> ep;
       ep.Y = W.y.p; ep.alpha = 1.0;
       ep.X = r2; ep.a = 1.0; ep.ca = mst.p + (size_t)MS_CVN * b;
       ep.Z = r1; ep.b = 1.0; ep.cb = mst.p + (size_t)MS_CR1 * b;
       ep.xn = Djac.p;
       ep.xcol = mst.p + (size_t)MS_CV * b;
       ep.dot_partial = partial.p;
-      int nbk = apply<T>(r2, b, ep);
+      int nbk = f32op ? apply_f32op(r2, b, ep) : apply<double>(r2, b, ep);
       launch_minres_alpha(b, nbk, partial.p, mst.p, s);
-      nbk = launch_minres_lanczos<T>(N, b, W.y.p, r2, r1, Djac.p, mst.p, partial.p, s);
+      nbk = launch_minres_lanczos<double>(N, b, W.y.p, r2, r1, Djac.p, mst.p, partial.p, s);
       launch_minres_givens(b, nbk, partial.p, adj.dnegEig.p, mst.p, s);
-      launch_minres_update<T>(N, b, r2, Djac.p, w1, w, x, mst.p, s);
+      launch_minres_update<double>(N, b, r2, Djac.p, w1, w, x, mst.p, s);
       std::swap(r1, r2);
       std::swap(w1, w);
     };
     // check_every steps between host convergence checks, captured once per
-    // solve as a HIP graph and replayed (kernels only: no memset or copy
-    // nodes); the apply scratch is sized before the capture.
+    // solve and operator precision as a HIP graph and replayed (kernels only:
+    // no memset or copy nodes); the apply scratch is sized before capture.
     const bool useGraph = d.minres_graph != 0 && check % 2 == 0;
-    if constexpr (std::is_same<T, double>::value) Yc64.resize(cell_scratch_elems(g, b));
-    else Yc32.resize(cell_scratch_elems(g, b));
-    hipGraphExec_t exec = nullptr;
+    Yc64.resize(cell_scratch_elems(g, b));
+    if (canRelax) Yc32.resize(cell_scratch_elems(g, b));
+    hipGraphExec_t execs[2] = {nullptr, nullptr};
     struct ExecGuard {
-      hipGraphExec_t& e;
-      ~ExecGuard() { if (e) (void)hipGraphExecDestroy(e); }
-    } guard{exec};
+      hipGraphExec_t* e;
+      ~ExecGuard() {
+        for (int i = 0; i < 2; ++i)
+          if (e[i]) (void)hipGraphExecDestroy(e[i]);
+      }
+    } guard{execs};
     for (int k = 0; k < maxit;) {
       launch_copy_done(b, mst.p, mflags.p, s);
       IDFT_HIP(hipMemcpyAsync(flags.data(), mflags.p, b * sizeof(int), hipMemcpyDeviceToHost, s));
+      if (canRelax && !f32op)
+        IDFT_HIP(hipMemcpyAsync(phibar.data(), mst.p + (size_t)MS_PHIBAR * b, b * sizeof(double),
+                                hipMemcpyDeviceToHost, s));
       sync();
       bool all = true;
       for (int f : flags) all = all && f;
       if (all) break;
+      if (canRelax && !f32op) {
+        bool ready = true;
+        for (int j = 0; j < b; ++j)
+          if (!flags[j] && !(phibar[j] <= relax * tol[j])) ready = false;
+        if (ready) {
+          f32op = true;
+          if (verbose)
+            std::fprintf(stderr, "[idft] minres: FP32 cell operator from step %d\n", k);
+        }
+      }
       const int n = std::min(check, maxit - k);
       if (useGraph && n == check) {
+        hipGraphExec_t& exec = execs[f32op ? 1 : 0];
         if (!exec) {
           hipGraph_t graph = nullptr;
           IDFT_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -707,15 +734,13 @@ struct idft_ctx {
           const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
           (void)hipGraphDestroy(graph);
           IDFT_HIP(ie);
-        } else {
+        } else if (f32op) {
           // the capture counted its n steps; replays count theirs here
-          if constexpr (std::is_same<T, double>::value) {
-            st.applies_fp64 += n;
-            st.apply_cols_fp64 += (int64_t)n * b;
-          } else {
-            st.applies_fp32 += n;
-            st.apply_cols_fp32 += (int64_t)n * b;
-          }
+          st.applies_fp32 += n;
+          st.apply_cols_fp32 += (int64_t)n * b;
+        } else {
+          st.applies_fp64 += n;
+          st.apply_cols_fp64 += (int64_t)n * b;
         }
         IDFT_HIP(hipGraphLaunch(exec, s));
       } else {
@@ -749,54 +774,54 @@ struct idft_ctx {
     launch_lincomb<double>(t.N, b, xp, 0.0, nullptr, 0.0, nullptr, nullptr, 0.0, nullptr, nullptr, s);
     std::vector<int> total(b, 0);
     if (!mixed) {
-      total = minres_core<double>(rhs.p, xp, b, std::vector<double>(b, tol), maxit);
+      total = minres_core(rhs.p, xp, b, std::vector<double>(b, tol), maxit, 0.0);
       if (verbose) {
         const int mx = *std::max_element(total.begin(), total.end());
         std::fprintf(stderr, "[idft] minres fp64: b=%d tol=%.3e max_its=%d\n", b, tol, mx);
       }
     } else {
-      // FP32 inner MINRES on the FP32 cell operator, FP64 iterative refinement:
-      // r = b - (H~ - eps) x in FP64, inner solve to rel. inner_tol, x += delta.
+      // FP64 recurrence with the relaxed (FP64, then FP32) cell operator.  The
+      // true residual r = b - (H~ - eps) x, in FP64, then decides per column
+      // whether a further sweep solves for the correction (x += delta).
       rref.resize(NB(b));
+      xdelta.resize(NB(b));
       double* r = rref.p;
-      mw32.ensure(NB(b));
       int used = 0, sweeps = 0;
+      std::vector<double> tl(b, tol);
       while (used < maxit) {
-        {
-          ApplyEpilogue<double> ep;  // r = rhs - (H~ - eps) x
-          ep.Y = r; ep.alpha = -1.0;
-          ep.X = xp; ep.a = -1.0; ep.ca = adj.dnegEig.p;
-          ep.Z = rhs.p; ep.b = 1.0;
-          apply<double>(xp, b, ep);
-        }
+        const bool first = sweeps == 0;
+        std::vector<int> it = minres_core(first ? rhs.p : r, first ? xp : xdelta.p, b, tl,
+                                          maxit - used, d.relax_ratio);
+        if (!first)
+          launch_lincomb<double>(t.N, b, xp, 1.0, xdelta.p, 1.0, nullptr, nullptr, 0.0, nullptr,
+                                 nullptr, s);
+        int mxi = 0;
+        for (int j = 0; j < b; ++j) { total[j] += it[j]; mxi = std::max(mxi, it[j]); }
+        used += std::max(mxi, 1);
+        sweeps++;
+        ApplyEpilogue<double> ep;  // r = rhs - (H~ - eps) x
+        ep.Y = r; ep.alpha = -1.0;
+        ep.X = xp; ep.a = -1.0; ep.ca = adj.dnegEig.p;
+        ep.Z = rhs.p; ep.b = 1.0;
+        apply<double>(xp, b, ep);
         std::vector<double> rn = coldot(r, r, b, Djac.p);
-        std::vector<double> tl(b);
         bool all = true;
         double rmax = 0.0;
         for (int j = 0; j < b; ++j) {
           const double nrm = std::sqrt(std::max(rn[j], 0.0));
           if (!std::isfinite(nrm)) throw Error(IDFT_ENAN, "non-finite refinement residual");
           rmax = std::max(rmax, nrm);
-          if (nrm <= tol) tl[j] = INFINITY;  // converged column: start as done
-          else { tl[j] = std::max(tol, d.inner_tol * nrm); all = false; }
+          if (nrm <= tol) tl[j] = INFINITY;  // converged column: starts as done
+          else { tl[j] = tol; all = false; }
         }
         if (verbose)
-          std::fprintf(stderr, "[idft] minres mixed: sweep %d used %d max |r|_D %.3e tol %.3e\n",
+          std::fprintf(stderr, "[idft] minres mixed: sweep %d steps %d true max |r|_D %.3e tol %.3e\n",
                        sweeps, used, rmax, tol);
         if (all) break;
-        launch_convert_d2f((int64_t)NB(b), mw32.rhs.p, r, s);
-        x32.resize(NB(b));
-        launch_lincomb<float>(t.N, b, x32.p, 0.0, nullptr, 0.0, nullptr, nullptr, 0.0, nullptr, nullptr, s);
-        std::vector<int> it = minres_core<float>(mw32.rhs.p, x32.p, b, tl, maxit - used);
-        // x += delta (FP32 increment, FP64 accumulation)
-        launch_convert_f2d((int64_t)NB(b), r, x32.p, s);
-        launch_lincomb<double>(t.N, b, xp, 1.0, r, 1.0, nullptr, nullptr, 0.0, nullptr, nullptr, s);
-        int mx = 0;
-        for (int j = 0; j < b; ++j) { total[j] += it[j]; mx = std::max(mx, it[j]); }
-        used += std::max(mx, 1);
-        sweeps++;
+        launch_lincomb<double>(t.N, b, xdelta.p, 0.0, nullptr, 0.0, nullptr, nullptr, 0.0, nullptr,
+                               nullptr, s);
       }
-      st.refinements += sweeps;
+      st.refinements += sweeps - 1;
     }
     int mx = 0;
     int64_t tot = 0;

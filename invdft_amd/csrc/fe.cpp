@@ -87,7 +87,7 @@ double lagrange_d(const std::vector<double>& xs, int a, double x) {
 bool desc_valid(const idft_desc* d) {
   return d && d->n_cells >= 1 && d->degree >= 2 && d->degree <= 5 && d->half_width > 0 &&
          d->n_eig >= 1 && d->n_eig <= kMaxCols &&
-         (d->precision != IDFT_MIXED || (d->inner_tol > 0.0 && d->inner_tol < 1.0));
+         d->relax_ratio >= 0.0;
 }
 
 void Tables::init(const idft_desc& d) {
@@ -253,7 +253,7 @@ void idft_desc_default(idft_desc* d) {
   d->frac_occ_tol = 1e-8; d->additional_states = 0; d->block_size = 0;
   d->rho_tol_constraints = 1e-6; d->gradient_mode = IDFT_GRADIENT_REFERENCE;
   d->lanczos_steps = 12; d->refine_every = 40; d->check_every = 8; d->seed = 1;
-  d->minres_graph = 1; d->inner_tol = 1e-5;
+  d->minres_graph = 1; d->relax_ratio = 1e3;
 }
 
 int idft_geometry_sizes(const idft_desc* d, idft_sizes* s) {

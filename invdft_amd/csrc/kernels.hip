@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "common.h"
 
@@ -137,17 +138,19 @@ __device__ inline typename Vec4<T>::type fetch_tile(const T* __restrict__ Hc, in
 // second read is served by the caches), KPF k-groups in flight per wave.
 // Output: Yc[e][l][j] = (H_e (S X)_e)[l][j].  Work-groups are dealt to the 8
 // XCDs in contiguous runs of cells.
-template <typename T, int P, int NCT>
+template <typename T, typename TV, int P, int NCT>
 __global__ __launch_bounds__(kThreads) void k_cell_gemm(Geo g, const T* __restrict__ H,
                                                         const T* __restrict__ S,
-                                                        const T* __restrict__ X,
+                                                        const TV* __restrict__ X,
                                                         const double* __restrict__ xn,
                                                         const double* __restrict__ xcol,
                                                         T* __restrict__ Yc, int B, int cpw) {
   constexpr int NE = ne_of(P), NEP = nep_of(P), RT = NEP / 16, BT = NCT * 16;
   constexpr int NTILE = ntiles_of(P);
   constexpr int LDB = ldb_of(BT);
-  constexpr int KPF = 3;
+  // k-groups of H_e in flight per wave: narrow tiles issue few MFMAs per
+  // load, so they keep more loads outstanding to cover the memory latency
+  constexpr int KPF = NCT <= 2 ? 6 : 3;
   __shared__ T xs[NEP * LDB];
   const int64_t nwg = (g.ncell + cpw - 1) / cpw;
   const int64_t per = (nwg + 7) / 8;
@@ -169,9 +172,16 @@ __global__ __launch_bounds__(kThreads) void k_cell_gemm(Geo g, const T* __restri
       T v = T(0);
       if (l < NE && j < bw) {
         const int64_t n = cell_node(g, cx, cy, cz, l);
-        v = S[n] * X[n * B + c0 + j];
-        if (xn) v *= (T)xn[n];
-        if (xcol) v *= (T)xcol[c0 + j];
+        if constexpr (std::is_same<T, TV>::value) {
+          v = S[n] * X[n * B + c0 + j];
+          if (xn) v *= (T)xn[n];
+          if (xcol) v *= (T)xcol[c0 + j];
+        } else {  // wider input vectors: scale in FP64, round once
+          double w = (double)S[n] * (double)X[n * B + c0 + j];
+          if (xn) w *= xn[n];
+          if (xcol) w *= xcol[c0 + j];
+          v = (T)w;
+        }
       }
       xs[l * LDB + j] = v;
     }
@@ -230,9 +240,9 @@ __global__ __launch_bounds__(kThreads) void k_cell_gemm(Geo g, const T* __restri
 // (iy, iz) in [by * R, by * R + R) and x positions [bx * per, bx * per + per)
 // of each row; thread t owns column t % B and x offset t / B for its whole
 // life.
-template <typename T, int P>
+template <typename TY, typename T, int P>
 __global__ __launch_bounds__(kThreads) void k_cell_gather(
-    Geo g, uint32_t B, uint32_t R, const T* __restrict__ Yc, const T* __restrict__ S,
+    Geo g, uint32_t B, uint32_t R, const TY* __restrict__ Yc, const T* __restrict__ S,
     T* __restrict__ Y, double alpha, double beta, const T* __restrict__ X, double a,
     const double* __restrict__ ca, const T* __restrict__ Z, double b,
     const double* __restrict__ cb, const T* __restrict__ W, double c,
@@ -281,11 +291,11 @@ __global__ __launch_bounds__(kThreads) void k_cell_gather(
       }
       // all (up to 8) contributions are loaded before the first is summed:
       // predicated, fully unrolled, so the loads are in flight together
-      T c8[8];
+      TY c8[8];
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
         const int kz = k >> 2, ky = (k >> 1) & 1, kx = k & 1;
-        c8[k] = T(0);
+        c8[k] = TY(0);
         if (kz < nz && ky < ny && kx < nx) {
           const uint64_t e = ((uint64_t)czs[kz] * nc + cys[ky]) * nc + cxs[kx];
           const uint32_t l = (azs[kz] * n1 + ays[ky]) * n1 + axs[kx];
@@ -1095,8 +1105,8 @@ void launch_build_hamiltonian(const Geo& g, const DevTables& t, double hh, const
   IDFT_HIP(hipGetLastError());
 }
 
-template <typename T, int P>
-static void cell_gemm_p(const Geo& g, const T* H, const T* S, const T* X, const double* xn,
+template <typename T, typename TV, int P>
+static void cell_gemm_p(const Geo& g, const T* H, const T* S, const TV* X, const double* xn,
                         const double* xcol, T* Yc, int B, const GemmShape& shape, hipStream_t s) {
   const int vt = std::min(shape.vectors_per_tile, ((B + 15) / 16) * 16);
   const int nct = std::max(1, std::min(5, vt / 16));
@@ -1105,11 +1115,11 @@ static void cell_gemm_p(const Geo& g, const T* H, const T* S, const T* X, const 
   dim3 grid((unsigned)(8 * ((nwg + 7) / 8)), (unsigned)((B + 16 * nct - 1) / (16 * nct))),
       block(kThreads);
   switch (nct) {
-    case 1: k_cell_gemm<T, P, 1><<<grid, block, 0, s>>>(g, H, S, X, xn, xcol, Yc, B, cpw); break;
-    case 2: k_cell_gemm<T, P, 2><<<grid, block, 0, s>>>(g, H, S, X, xn, xcol, Yc, B, cpw); break;
-    case 3: k_cell_gemm<T, P, 3><<<grid, block, 0, s>>>(g, H, S, X, xn, xcol, Yc, B, cpw); break;
-    case 4: k_cell_gemm<T, P, 4><<<grid, block, 0, s>>>(g, H, S, X, xn, xcol, Yc, B, cpw); break;
-    case 5: k_cell_gemm<T, P, 5><<<grid, block, 0, s>>>(g, H, S, X, xn, xcol, Yc, B, cpw); break;
+    case 1: k_cell_gemm<T, TV, P, 1><<<grid, block, 0, s>>>(g, H, S, X, xn, xcol, Yc, B, cpw); break;
+    case 2: k_cell_gemm<T, TV, P, 2><<<grid, block, 0, s>>>(g, H, S, X, xn, xcol, Yc, B, cpw); break;
+    case 3: k_cell_gemm<T, TV, P, 3><<<grid, block, 0, s>>>(g, H, S, X, xn, xcol, Yc, B, cpw); break;
+    case 4: k_cell_gemm<T, TV, P, 4><<<grid, block, 0, s>>>(g, H, S, X, xn, xcol, Yc, B, cpw); break;
+    case 5: k_cell_gemm<T, TV, P, 5><<<grid, block, 0, s>>>(g, H, S, X, xn, xcol, Yc, B, cpw); break;
   }
 }
 
@@ -1141,10 +1151,10 @@ void launch_cell_gemm(const Geo& g, const T* H, const T* S, const T* X, const do
                       int B, const GemmShape& shape, hipStream_t s) {
   if (B < 1 || B > kMaxCols) throw Error(IDFT_EINVAL, "block width out of range");
   switch (g.p) {
-    case 2: cell_gemm_p<T, 2>(g, H, S, X, nullptr, xcol, Yc, B, shape, s); break;
-    case 3: cell_gemm_p<T, 3>(g, H, S, X, nullptr, xcol, Yc, B, shape, s); break;
-    case 4: cell_gemm_p<T, 4>(g, H, S, X, nullptr, xcol, Yc, B, shape, s); break;
-    case 5: cell_gemm_p<T, 5>(g, H, S, X, nullptr, xcol, Yc, B, shape, s); break;
+    case 2: cell_gemm_p<T, T, 2>(g, H, S, X, nullptr, xcol, Yc, B, shape, s); break;
+    case 3: cell_gemm_p<T, T, 3>(g, H, S, X, nullptr, xcol, Yc, B, shape, s); break;
+    case 4: cell_gemm_p<T, T, 4>(g, H, S, X, nullptr, xcol, Yc, B, shape, s); break;
+    case 5: cell_gemm_p<T, T, 5>(g, H, S, X, nullptr, xcol, Yc, B, shape, s); break;
     default: throw Error(IDFT_EINVAL, "degree not supported by the HIP kernels (2..5)");
   }
   IDFT_HIP(hipGetLastError());
@@ -1154,15 +1164,15 @@ template void launch_cell_gemm<float>(const Geo&, const float*, const float*, co
 template void launch_cell_gemm<double>(const Geo&, const double*, const double*, const double*,
                                        const double*, double*, int, const GemmShape&, hipStream_t);
 
-template <typename T>
-int launch_cell_apply(const Geo& g, const T* H, const T* S, const T* X, T* Yc, int B,
-                      const ApplyEpilogue<T>& ep, const GemmShape& shape, hipStream_t s) {
+template <typename T, typename TV>
+int launch_cell_apply(const Geo& g, const T* H, const T* Sg, const TV* S, const TV* X, T* Yc,
+                      int B, const ApplyEpilogue<TV>& ep, const GemmShape& shape, hipStream_t s) {
   if (B < 1 || B > kMaxCols) throw Error(IDFT_EINVAL, "block width out of range");
   switch (g.p) {
-    case 2: cell_gemm_p<T, 2>(g, H, S, X, ep.xn, ep.xcol, Yc, B, shape, s); break;
-    case 3: cell_gemm_p<T, 3>(g, H, S, X, ep.xn, ep.xcol, Yc, B, shape, s); break;
-    case 4: cell_gemm_p<T, 4>(g, H, S, X, ep.xn, ep.xcol, Yc, B, shape, s); break;
-    case 5: cell_gemm_p<T, 5>(g, H, S, X, ep.xn, ep.xcol, Yc, B, shape, s); break;
+    case 2: cell_gemm_p<T, TV, 2>(g, H, Sg, X, ep.xn, ep.xcol, Yc, B, shape, s); break;
+    case 3: cell_gemm_p<T, TV, 3>(g, H, Sg, X, ep.xn, ep.xcol, Yc, B, shape, s); break;
+    case 4: cell_gemm_p<T, TV, 4>(g, H, Sg, X, ep.xn, ep.xcol, Yc, B, shape, s); break;
+    case 5: cell_gemm_p<T, TV, 5>(g, H, Sg, X, ep.xn, ep.xcol, Yc, B, shape, s); break;
     default: throw Error(IDFT_EINVAL, "degree not supported by the HIP kernels (2..5)");
   }
   IDFT_HIP(hipGetLastError());
@@ -1172,7 +1182,7 @@ int launch_cell_apply(const Geo& g, const T* H, const T* S, const T* X, T* Yc, i
   gather_grid(g, B, shape.gather_rows, gx, gy, R);
   dim3 grid(gx, gy);
 #define IDFT_GATHER(PP)                                                                       \
-  k_cell_gather<T, PP><<<grid, kThreads, 0, s>>>(g, (uint32_t)B, R, Yc, S, ep.Y, ep.alpha,   \
+  k_cell_gather<T, TV, PP><<<grid, kThreads, 0, s>>>(g, (uint32_t)B, R, Yc, S, ep.Y, ep.alpha, \
                                                  ep.beta, ep.X, ep.a, ep.ca, ep.Z, ep.b, ep.cb, \
                                                  ep.W, ep.c, ep.cc, X, ep.xn, ep.xcol, ep.V,  \
                                                  ep.dot_partial)
@@ -1186,12 +1196,18 @@ int launch_cell_apply(const Geo& g, const T* H, const T* S, const T* X, T* Yc, i
   IDFT_HIP(hipGetLastError());
   return (int)(gx * gy);
 }
-template int launch_cell_apply<float>(const Geo&, const float*, const float*, const float*, float*,
-                                      int, const ApplyEpilogue<float>&, const GemmShape&,
-                                      hipStream_t);
-template int launch_cell_apply<double>(const Geo&, const double*, const double*, const double*,
-                                       double*, int, const ApplyEpilogue<double>&,
-                                       const GemmShape&, hipStream_t);
+template int launch_cell_apply<float, float>(const Geo&, const float*, const float*, const float*,
+                                             const float*, float*, int,
+                                             const ApplyEpilogue<float>&, const GemmShape&,
+                                             hipStream_t);
+template int launch_cell_apply<double, double>(const Geo&, const double*, const double*,
+                                               const double*, const double*, double*, int,
+                                               const ApplyEpilogue<double>&, const GemmShape&,
+                                               hipStream_t);
+template int launch_cell_apply<float, double>(const Geo&, const float*, const float*,
+                                              const double*, const double*, float*, int,
+                                              const ApplyEpilogue<double>&, const GemmShape&,
+                                              hipStream_t);
 
 template <typename T>
 void launch_lincomb(int64_t N, int B, T* Y, double beta, const T* X, double a, const double* ca,

@@ -1213,7 +1213,7 @@ void idft_desc_default(idft_desc* d) {
   d->rho_tol_constraints = 1e-6; d->gradient_mode = IDFT_GRADIENT_REFERENCE;
   d->lanczos_steps = 12; d->refine_every = 40; d->check_every = 8; d->seed = 1;
   d->minres_graph = 1;  // HIP launch option; no effect here
-  d->inner_tol = 1e-5;  // mixed-precision option; no effect here
+  d->relax_ratio = 1e3;  // mixed-precision option; no effect here
 }
 
 static bool desc_ok(const idft_desc* d) {
