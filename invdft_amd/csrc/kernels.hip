@@ -109,16 +109,25 @@ template <> struct Vec4<double> { typedef double4 type; };
 
 __host__ __device__ constexpr int ldb_of(int bt) { return (bt % 32 == 16) ? bt : bt + 16; }
 
-// A operand of four MFMA k-steps for block row i, k-group k of H_e (Hc = the
-// cell's stored tiles).  k >= i: tile (i, k) as stored, one vector load.
-// k < i: tile (k, i) transposed -- lane l needs T[4s + l/16][l%16], which
-// sits in lane (4s + l/16) + 16 (l%4), slot (l%16)/4 of the stored tile:
-// four scalar loads from the same 16x16 tile.
+// Stored tile of block row i, k-group k of H_e (Hc = the cell's stored
+// tiles), one vector load per lane: tile (i, k) for k >= i, else its mirror
+// (k, i), which the caller transposes (lds_transpose).
 template <typename T, int RT>
 __device__ inline typename Vec4<T>::type fetch_tile(const T* __restrict__ Hc, int i, int k,
                                                      int lane) {
   typedef typename Vec4<T>::type vec_t;
-  if (k >= i) return reinterpret_cast<const vec_t*>(Hc)[tile_index(RT, i, k) * 64 + lane];
+  const int t = k >= i ? tile_index(RT, i, k) : tile_index(RT, k, i);
+  return reinterpret_cast<const vec_t*>(Hc)[t * 64 + lane];
+}
+
+// The transposed tile read straight from memory: lane l needs
+// T[4s + l/16][l%16], which sits in lane (4s + l/16) + 16 (l%4), slot
+// (l%16)/4 of the mirror tile (k, i) -- four scalar loads of the same 1 KiB
+// (FP32) tile.  Used where the LDS square would cost occupancy.
+template <typename T, int RT>
+__device__ inline typename Vec4<T>::type fetch_tile_transposed(const T* __restrict__ Hc, int i,
+                                                                int k, int lane) {
+  typedef typename Vec4<T>::type vec_t;
   const T* t = Hc + tile_index(RT, k, i) * 256 + ((lane & 15) >> 2);
   const int lb = (lane >> 4) + 16 * (lane & 3);
   vec_t v;
@@ -129,13 +138,39 @@ __device__ inline typename Vec4<T>::type fetch_tile(const T* __restrict__ Hc, in
   return v;
 }
 
+// A operand of the transposed tile: lane l holds m = T[l%16][4s + l/16]
+// (s = 0..3) and needs T[4s + l/16][l%16].  Through a per-wave 16 x 17
+// (padded) LDS square: write row-major, read column-major.  LDS operations of
+// one wave execute in order; the wavefront fences keep the compiler from
+// moving the reads above the writes.
+template <typename T>
+__device__ inline typename Vec4<T>::type lds_transpose(typename Vec4<T>::type m, T* sq,
+                                                        int lane) {
+  typedef typename Vec4<T>::type vec_t;
+  const int a = lane & 15, b = lane >> 4;
+  sq[a * 17 + b] = m.x;
+  sq[a * 17 + 4 + b] = m.y;
+  sq[a * 17 + 8 + b] = m.z;
+  sq[a * 17 + 12 + b] = m.w;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  vec_t t;
+  t.x = sq[b * 17 + a];
+  t.y = sq[(4 + b) * 17 + a];
+  t.z = sq[(8 + b) * 17 + a];
+  t.w = sq[(12 + b) * 17 + a];
+  return t;
+}
+
 // Phase 1 of the cell operator.  One work-group handles `cpw` consecutive
 // cells and the column tile [blockIdx.y * 16 NCT, +16 NCT) of the block.  Per
 // cell the tile of (S X)_e is gathered into LDS and each wave computes whole
 // 16-row blocks of H_e (S X)_e, streaming the tiles of its block row from
-// HBM (upper-triangle tiles as stored, lower-triangle ones as transposes of
-// their mirror tiles, which the mirror's own row pass reads too, so the
-// second read is served by the caches), KPF k-groups in flight per wave.
+// HBM (upper-triangle tiles as stored, lower-triangle ones as LDS transposes
+// of their mirror tiles, which the mirror's own row pass reads too, so the
+// second read is mostly served by the caches), KPF k-groups in flight per
+// wave.
 // Output: Yc[e][l][j] = (H_e (S X)_e)[l][j].  Work-groups are dealt to the 8
 // XCDs in contiguous runs of cells.
 template <typename T, typename TV, int P, int NCT>
@@ -148,10 +183,23 @@ __global__ __launch_bounds__(kThreads) void k_cell_gemm(Geo g, const T* __restri
   constexpr int NE = ne_of(P), NEP = nep_of(P), RT = NEP / 16, BT = NCT * 16;
   constexpr int NTILE = ntiles_of(P);
   constexpr int LDB = ldb_of(BT);
-  // k-groups of H_e in flight per wave: narrow tiles issue few MFMAs per
-  // load, so they keep more loads outstanding to cover the memory latency
-  constexpr int KPF = NCT <= 2 ? 6 : 3;
+  // k-groups of H_e in flight per wave (deeper prefetch costs occupancy and
+  // measured slower at p = 4, 5)
+  constexpr int KPF = 3;
+  // lower-triangle tiles through per-wave LDS transpose squares, unless
+  // these would push the work-group past half the 160 KiB LDS (two
+  // work-groups per CU), as for FP64 with 80-column tiles
+  constexpr bool kLdsT = (NEP * LDB + 4 * 272) * (int)sizeof(T) <= 81920 ||
+                         NEP * LDB * (int)sizeof(T) > 81920;
   __shared__ T xs[NEP * LDB];
+  __shared__ T tsq[kLdsT ? 4 : 1][kLdsT ? 16 * 17 : 1];
+  typedef typename Vec4<T>::type vec_t;
+  // stored tile (transposed by the caller) or the transposed tile itself
+  auto fetch = [](auto ldsT, const T* Hc, int i, int k, int lane) -> vec_t {
+    if constexpr (decltype(ldsT)::value) return fetch_tile<T, RT>(Hc, i, k, lane);
+    else return k >= i ? fetch_tile<T, RT>(Hc, i, k, lane)
+                       : fetch_tile_transposed<T, RT>(Hc, i, k, lane);
+  };
   const int64_t nwg = (g.ncell + cpw - 1) / cpw;
   const int64_t per = (nwg + 7) / 8;
   const int64_t wg = (blockIdx.x % 8) * per + blockIdx.x / 8;
@@ -162,7 +210,6 @@ __global__ __launch_bounds__(kThreads) void k_cell_gemm(Geo g, const T* __restri
   const int wave = tid / 64, lane = tid % 64;
   const int lr = lane % 16, lk = lane / 16;
   typedef typename Mfma<T>::acc_t acc_t;
-  typedef typename Vec4<T>::type vec_t;
   const int64_t e0 = wg * cpw, e1 = min((int64_t)g.ncell, e0 + cpw);
   for (int64_t e = e0; e < e1; ++e) {
     const int64_t cx = e % g.nc, cy = (e / g.nc) % g.nc, cz = e / ((int64_t)g.nc * g.nc);
@@ -195,15 +242,18 @@ __global__ __launch_bounds__(kThreads) void k_cell_gemm(Geo g, const T* __restri
       vec_t buf[KPF];
 #pragma unroll
       for (int kk = 0; kk < KPF; ++kk)
-        buf[kk] = kk < RT ? fetch_tile<T, RT>(Hc, rt, kk, lane) : vec_t{};
+        buf[kk] = kk < RT ? fetch(std::integral_constant<bool, kLdsT>{}, Hc, rt, kk, lane) : vec_t{};
 #pragma unroll 1
       for (int kg0 = 0; kg0 < RT; kg0 += KPF) {
 #pragma unroll
         for (int kk = 0; kk < KPF; ++kk) {
           const int kg = kg0 + kk;
           if (kg < RT) {
-            const vec_t a = buf[kk];
-            if (kg + KPF < RT) buf[kk] = fetch_tile<T, RT>(Hc, rt, kg + KPF, lane);
+            vec_t a = buf[kk];
+            if (kg + KPF < RT) buf[kk] = fetch(std::integral_constant<bool, kLdsT>{}, Hc, rt, kg + KPF, lane);
+            if constexpr (kLdsT) {
+              if (kg < rt) a = lds_transpose<T>(a, tsq[wave], lane);
+            }
             const T* xk = xs + (kg * 16 + lk) * LDB + lr;
             const T av[4] = {a.x, a.y, a.z, a.w};
 #pragma unroll
