@@ -11,9 +11,10 @@ target density is the FP64 ground state of the wells alone; v_xc starts at a
 seeded sum of 8 Gaussians).
 
 The mixed-precision run (FP32-MFMA cell operator, defect-corrected FP32
-Chebyshev filter, FP32 inner MINRES with FP64 refinement, FP64 everything
-else) is timed over K iterations after W warm-up iterations, then the FP64
-run on the same problem.  Every library call returns only after its work on
+Chebyshev filter, FP64 MINRES recurrence whose cell operator drops to FP32
+once the residual is near the tolerance, FP64 everything else) is timed over
+K iterations after W warm-up iterations, then the FP64 run on the same
+problem.  Every library call returns only after its work on
 the GPU stream has completed, so host wall-clock brackets are synchronised.
 """
 import argparse
@@ -30,10 +31,14 @@ sys.path.insert(0, HERE)
 METRIC = "s per BFGS iter (eigen+adjoint+force), mixed vs FP64 on GPU, force err ≤1e-6"
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the dominant kernel from two rocprofv3 --pmc runs
-# (FETCH_SIZE, WRITE_SIZE) of `bench.py --kernels-only` at the autotuned
-# shape, FETCH_SIZE scaled by the copy calibration; the summary with the
-# commands is profiles/cell_gemm_inverse_24c_p5_pmc.md.
-MEASURED_TRAFFIC = {}
+# (FETCH_SIZE, WRITE_SIZE) of `bench.py --kernels-only --shape 2 16 4 0` /
+# `... 2 16 0 1`, FETCH_SIZE scaled by the copy calibration (x1.926), WRITE_SIZE
+# as read; the summary with the commands is
+# profiles/cell_gemm_inverse_24c_p5_pmc.md.
+MEASURED_TRAFFIC = {
+    ("inverse_24c_p5", "k_cell_gemm<float,float,5,1>"): 2613e6,
+    ("inverse_24c_p5", "k_cell_symm<float,float,5,1>"): 1756e6,
+}
 
 
 def log(*a):
@@ -133,6 +138,51 @@ def kernels_only(lib, cfg, shape, reps):
           flush=True)
 
 
+def adjoint_only(lib, cfg, reps):
+    """The multivector adjoint solve alone (the reference's
+    TestMultiVectorAdjointProblem path): FP64 eigensolve of the wells plus
+    the seeded v_xc, then Jacobi-preconditioned MINRES on
+    (H~ - eps_i) p_i = b_i for every column, to a residual of 1e-10 relative
+    to |b|_D, timed over `reps` solves, FP64 and mixed operator."""
+    from invdft_amd import abi, synthetic
+    prob = synthetic.make_problem(lib, seed=1, target_tol=1e-10, **cfg)
+    out = {}
+    for name, prec in (("fp64", abi.FP64), ("mixed", abi.MIXED)):
+        ctx = prob.context(lib, precision=prec)
+        ctx.set_potential(prob.vxc0)
+        ev, _, _ = ctx.eigensolve(1e-10, 200)
+        occ, fermi = ctx.occupancies()
+        psi = ctx.eigenvectors()
+        u = prob.rho_target_q - 0.5 * ctx.density()
+        B = psi.shape[1]
+        pairs = [v for i in range(B) for j in range(B) if abs(ev[i] - ev[j]) < 0.002
+                 for v in (i, j)]
+        mpsi = ctx.mv(B, psi)
+        ctx.adj_update_input_psi(mpsi, occ, u, pairs, fermi, ev.copy())
+        x, rhs, drhs = ctx.mv(B), ctx.mv(B), ctx.mv(B)
+        ctx.adj_compute_rhs(None, x, rhs)
+        ctx.adj_precondition_jacobi(drhs, rhs)
+        bnorm = float(np.sqrt(np.max(np.sum(rhs.get() * drhs.get(), axis=0))))
+        tol = 1e-10 * bnorm
+        ctx.minres_solve(x, None, tol, 5000, distribute=True)  # warm-up
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            its = ctx.minres_solve(x, None, tol, 5000, distribute=True)
+        dt = (time.perf_counter() - t0) / reps
+        out[name] = {"s_per_solve": round(dt, 5), "iterations_max": int(np.max(its)),
+                     "columns": B, "occupied": int(np.sum(occ > 1e-8)),
+                     "ms_per_iteration": round(1e3 * dt / max(int(np.max(its)), 1), 4)}
+        log(f"  {name}: {out[name]}")
+        ctx.close()
+    print(json.dumps({"metric": "s per multivector adjoint MINRES solve (rel. residual 1e-10)",
+                      "value": out["fp64"]["s_per_solve"], "unit": "s/solve", "n_gpus": 1,
+                      "higher_is_better": False, "dtype": "f64", "data": "synthetic",
+                      "config": {"workload": "adjoint_" + str(cfg["n_cells"]) + "c_p"
+                                 + str(cfg["degree"]), "n_cells": cfg["n_cells"],
+                                 "degree": cfg["degree"]},
+                      "detail": out}), flush=True)
+
+
 def uneven_batch(lib, count, steps, workers):
     """Uneven batch of independent small inverse problems: problems/s and
     per-problem p50/p99 seconds for `steps` L-BFGS iterations each, mixed
@@ -173,8 +223,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the operator kernels at --shape (for counter runs)")
-    ap.add_argument("--shape", type=int, nargs=3, default=[1, 16, 0],
-                    metavar=("CELLS_PER_WG", "VECTORS_PER_TILE", "GATHER_ROWS"))
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="only the multivector adjoint MINRES solve of --config")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch MINRES steps one by one instead of replaying a HIP graph "
+                         "(for rocprofv3 --kernel-trace, whose graph interception crashes)")
+    ap.add_argument("--shape", type=int, nargs=4, default=[1, 16, 0, 0],
+                    metavar=("CELLS_PER_WG", "VECTORS_PER_TILE", "GATHER_ROWS", "TILE_PASS"))
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--batch-count", type=int, default=96,
                     help="problems of --config uneven_batch")
@@ -197,11 +252,16 @@ def main():
     if args.kernels_only:
         kernels_only(lib, cfg, args.shape, args.reps)
         return
+    if args.adjoint_only:
+        adjoint_only(lib, cfg, args.reps)
+        return
     log(f"config {args.config}: {cfg}")
     t0 = time.perf_counter()
     prob = synthetic.make_problem(lib, seed=2, target_tol=1e-10, **cfg)
     log(f"setup (target density by FP64 eigensolve on the GPU): {time.perf_counter() - t0:.1f} s")
     tol_kw = dict(cheb_tol=args.cheb_tol, initial_tol_cheb=args.cheb_tol)
+    if args.no_graph:
+        tol_kw["minres_graph"] = 0
 
     log("mixed precision run")
     cm, mixed = run_bfgs(lib, prob, abi.MIXED, args.warmup, args.steps, tol_kw)
@@ -244,7 +304,10 @@ def main():
     bytes_gemm = gemm_bytes(4)
     bytes_op = s.n_cells * nh * 4 + 2 * s.n_nodes * B * 4
     achieved = bytes_gemm / (ms_gemm * 1e-3) / 1e9
-    kname = f"k_cell_gemm<float,{prob.desc.degree},{(B + 15) // 16}>"
+    tuned32 = mixed["tuned"]["fp32"]
+    nct = min(5, max(1, min(tuned32["vectors_per_tile"], (B + 15) // 16 * 16) // 16))
+    kname = (f"k_cell_{'symm' if tuned32['tile_pass'] else 'gemm'}<float,float,"
+             f"{prob.desc.degree},{nct}>")
     traffic = MEASURED_TRAFFIC.get((args.config, kname))
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,

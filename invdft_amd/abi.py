@@ -442,23 +442,25 @@ class Context:
         self.bfgs_times_ms = th[: nd.value + 1]
         return lh[: nd.value + 1], gh[: nd.value + 1], nd.value
 
-    def set_apply_shape(self, precision, cells_per_wg, vectors_per_tile, gather_rows=0):
+    def set_apply_shape(self, precision, cells_per_wg, vectors_per_tile, gather_rows=0,
+                        tile_pass=0):
         fn = self.lib.idft_set_apply_shape
         fn.restype = C.c_int
-        fn.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32]
-        self._check(fn(self.h, precision, cells_per_wg, vectors_per_tile, gather_rows))
+        fn.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32]
+        self._check(fn(self.h, precision, cells_per_wg, vectors_per_tile, tile_pass,
+                       gather_rows))
 
     def autotune_apply(self, ncols, precision=FP64, reps=5):
         """Pick the fastest operator launch shape (HIP library only)."""
-        cp, vt, gr = C.c_int32(0), C.c_int32(0), C.c_int32(0)
+        cp, vt, tp, gr = C.c_int32(0), C.c_int32(0), C.c_int32(0), C.c_int32(0)
         ms = np.zeros(1)
         fn = self.lib.idft_autotune_apply
         fn.restype = C.c_int
-        fn.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, _IP, _IP, _IP, _DP]
-        self._check(fn(self.h, ncols, precision, reps, C.byref(cp), C.byref(vt), C.byref(gr),
-                       _dp(ms)))
-        return dict(cells_per_wg=cp.value, vectors_per_tile=vt.value, gather_rows=gr.value,
-                    ms=float(ms[0]))
+        fn.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, _IP, _IP, _IP, _IP, _DP]
+        self._check(fn(self.h, ncols, precision, reps, C.byref(cp), C.byref(vt), C.byref(tp),
+                       C.byref(gr), _dp(ms)))
+        return dict(cells_per_wg=cp.value, vectors_per_tile=vt.value, tile_pass=tp.value,
+                    gather_rows=gr.value, ms=float(ms[0]))
 
     def profile_apply(self, ncols, precision=FP64, reps=10):
         """Device ms per Hamiltonian application and for its MFMA cell-product

@@ -146,6 +146,7 @@ struct GemmShape {
   int cells_per_wg = 1;       // cells processed in sequence by one work-group
   int vectors_per_tile = 80;  // columns per launch tile (rounded to 16; >= B: one pass)
   int gather_rows = 0;        // node rows per gather work-group (0: about kGatherBlocks groups)
+  int tile_pass = 0;          // 1: one pass over the stored tiles (16-column tiles; FP32 also 32)
 };
 // T: cell matrices, cell scratch Yc and MFMA arithmetic; TV: the node-major
 // vectors (Xin and the epilogue's blocks).  Sg (type T) scales the input of

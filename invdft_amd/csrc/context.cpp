@@ -1377,21 +1377,22 @@ int idft_bfgs_set_checkpoint(idft_ctx* c, const char* path, int32_t every, int32
 }
 
 int idft_set_apply_shape(idft_ctx* c, int32_t prec, int32_t cells_per_wg,
-                         int32_t vectors_per_tile, int32_t gather_rows) {
+                         int32_t vectors_per_tile, int32_t tile_pass, int32_t gather_rows) {
   if (!c || cells_per_wg < 1 || cells_per_wg > 64 || vectors_per_tile < 16 ||
       vectors_per_tile > kMaxCols || vectors_per_tile % 16 || gather_rows < 0 ||
-      gather_rows > 4096)
+      gather_rows > 4096 || tile_pass < 0 || tile_pass > 1)
     return IDFT_EINVAL;
   GemmShape& t = prec == IDFT_MIXED ? c->shape32 : c->shape64;
   t.cells_per_wg = cells_per_wg;
   t.vectors_per_tile = vectors_per_tile;
+  t.tile_pass = tile_pass;
   t.gather_rows = gather_rows;
   return IDFT_OK;
 }
 
 int idft_autotune_apply(idft_ctx* c, int32_t ncols, int32_t prec, int32_t reps,
-                        int32_t* cells_per_wg, int32_t* vectors_per_tile, int32_t* gather_rows,
-                        double* ms_best) {
+                        int32_t* cells_per_wg, int32_t* vectors_per_tile, int32_t* tile_pass,
+                        int32_t* gather_rows, double* ms_best) {
   if (!c || ncols < 1 || ncols > kMaxCols || reps < 1) return IDFT_EINVAL;
   if (!c->haveH) { c->err = "no Hamiltonian"; return IDFT_ESTATE; }
   const bool f32 = prec == IDFT_MIXED;
@@ -1414,11 +1415,15 @@ int idft_autotune_apply(idft_ctx* c, int32_t ncols, int32_t prec, int32_t reps,
       if (ms < bestMs) { bestMs = ms; best = target; }
     };
     target.gather_rows = 0;
+    const int tpMax = f32 ? 32 : 16;  // widest tile of the tile-pass kernel
     for (int vt : widths) {
-      for (int cp : {1, 2, 4, 8}) {
-        target.cells_per_wg = cp;
-        target.vectors_per_tile = vt;
-        trial();
+      for (int tp = 0; tp <= (vt <= tpMax ? 1 : 0); ++tp) {
+        for (int cp : {1, 2, 4, 8}) {
+          target.cells_per_wg = cp;
+          target.vectors_per_tile = vt;
+          target.tile_pass = tp;
+          trial();
+        }
       }
     }
     target = best;
@@ -1429,6 +1434,7 @@ int idft_autotune_apply(idft_ctx* c, int32_t ncols, int32_t prec, int32_t reps,
     target = best;
     if (cells_per_wg) *cells_per_wg = best.cells_per_wg;
     if (vectors_per_tile) *vectors_per_tile = best.vectors_per_tile;
+    if (tile_pass) *tile_pass = best.tile_pass;
     if (gather_rows) *gather_rows = best.gather_rows;
     if (ms_best) *ms_best = bestMs;
   });

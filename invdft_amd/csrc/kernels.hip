@@ -163,6 +163,35 @@ __device__ inline typename Vec4<T>::type lds_transpose(typename Vec4<T>::type m,
   return t;
 }
 
+// The cell's input tile in LDS: xs[l][j] = S[n] xn[n] X[n][c0 + j] xcol[c0 + j]
+// (node n of local index l; rows l >= NE and columns j >= bw are zero).
+template <typename T, typename TV, int P, int BT, int LDB>
+__device__ inline void load_cell_input(const Geo& g, int64_t cx, int64_t cy, int64_t cz,
+                                       const T* __restrict__ S, const TV* __restrict__ X,
+                                       const double* __restrict__ xn,
+                                       const double* __restrict__ xcol, int B, int c0, int bw,
+                                       T* xs) {
+  constexpr int NE = ne_of(P), NEP = nep_of(P);
+  for (int i = threadIdx.x; i < NEP * BT; i += kThreads) {
+    const int l = i / BT, j = i % BT;
+    T v = T(0);
+    if (l < NE && j < bw) {
+      const int64_t n = cell_node(g, cx, cy, cz, l);
+      if constexpr (std::is_same<T, TV>::value) {
+        v = S[n] * X[n * B + c0 + j];
+        if (xn) v *= (T)xn[n];
+        if (xcol) v *= (T)xcol[c0 + j];
+      } else {  // wider input vectors: scale in FP64, round once
+        double w = (double)S[n] * (double)X[n * B + c0 + j];
+        if (xn) w *= xn[n];
+        if (xcol) w *= xcol[c0 + j];
+        v = (T)w;
+      }
+    }
+    xs[l * LDB + j] = v;
+  }
+}
+
 // Phase 1 of the cell operator.  One work-group handles `cpw` consecutive
 // cells and the column tile [blockIdx.y * 16 NCT, +16 NCT) of the block.  Per
 // cell the tile of (S X)_e is gathered into LDS and each wave computes whole
@@ -214,24 +243,7 @@ __global__ __launch_bounds__(kThreads) void k_cell_gemm(Geo g, const T* __restri
   for (int64_t e = e0; e < e1; ++e) {
     const int64_t cx = e % g.nc, cy = (e / g.nc) % g.nc, cz = e / ((int64_t)g.nc * g.nc);
     if (e > e0) __syncthreads();  // previous cell's tiles fully consumed
-    for (int i = tid; i < NEP * BT; i += kThreads) {
-      const int l = i / BT, j = i % BT;
-      T v = T(0);
-      if (l < NE && j < bw) {
-        const int64_t n = cell_node(g, cx, cy, cz, l);
-        if constexpr (std::is_same<T, TV>::value) {
-          v = S[n] * X[n * B + c0 + j];
-          if (xn) v *= (T)xn[n];
-          if (xcol) v *= (T)xcol[c0 + j];
-        } else {  // wider input vectors: scale in FP64, round once
-          double w = (double)S[n] * (double)X[n * B + c0 + j];
-          if (xn) w *= xn[n];
-          if (xcol) w *= xcol[c0 + j];
-          v = (T)w;
-        }
-      }
-      xs[l * LDB + j] = v;
-    }
+    load_cell_input<T, TV, P, BT, LDB>(g, cx, cy, cz, S, X, xn, xcol, B, c0, bw, xs);
     __syncthreads();
     const T* Hc = H + (size_t)e * NTILE * 256;
     T* yc = Yc + (size_t)e * NE * B + c0;
@@ -276,6 +288,133 @@ __global__ __launch_bounds__(kThreads) void k_cell_gemm(Geo g, const T* __restri
           if (r < NE) yc[(size_t)r * B + col] = acc[ct][i];
         }
       }
+    }
+  }
+}
+
+// Compile-time row / column of stored tile t (row-major upper triangle).
+__host__ __device__ constexpr int tile_row(int rt_count, int t) {
+  int r = 0;
+  while (r + 1 < rt_count && tile_index(rt_count, r + 1, r + 1) <= t) ++r;
+  return r;
+}
+__host__ __device__ constexpr int tile_col(int rt_count, int t) {
+  return tile_row(rt_count, t) + (t - tile_index(rt_count, tile_row(rt_count, t),
+                                                 tile_row(rt_count, t)));
+}
+
+template <int I>
+using ic = std::integral_constant<int, I>;
+template <int B0, int E0, typename F>
+__device__ inline void static_for(F&& f) {
+  if constexpr (B0 < E0) {
+    f(ic<B0>{});
+    static_for<B0 + 1, E0>(f);
+  }
+}
+
+// Phase 1, tile pass: every stored tile of H_e is read from memory once.
+// Wave w owns the contiguous quarter [T0, T1) of the cell's NT stored tiles
+// (a compile-time list, so every accumulator index is static), adds
+// T (S X)_k to rows r and, for r < k, T^T (S X)_r (LDS transpose) to rows k,
+// keeping partial sums of all RT row blocks; the four waves' partials are
+// then added in LDS in wave order (deterministic) and written to Yc.  For
+// narrow column tiles (the accumulators hold RT x NCT MFMA tiles per wave).
+template <typename T, typename TV, int P, int NCT>
+__global__ __launch_bounds__(kThreads) void k_cell_symm(Geo g, const T* __restrict__ H,
+                                                        const T* __restrict__ S,
+                                                        const TV* __restrict__ X,
+                                                        const double* __restrict__ xn,
+                                                        const double* __restrict__ xcol,
+                                                        T* __restrict__ Yc, int B, int cpw) {
+  constexpr int NE = ne_of(P), NEP = nep_of(P), RT = NEP / 16, BT = NCT * 16;
+  constexpr int NTILE = ntiles_of(P);
+  constexpr int LDB = ldb_of(BT);
+  constexpr int KPF = 3;
+  __shared__ T xs[NEP * LDB];  // the input tile, then the partial-sum buffer
+  __shared__ T tsq[4][16 * 17];
+  typedef typename Vec4<T>::type vec_t;
+  typedef typename Mfma<T>::acc_t acc_t;
+  const int64_t nwg = (g.ncell + cpw - 1) / cpw;
+  const int64_t per = (nwg + 7) / 8;
+  const int64_t wg = (blockIdx.x % 8) * per + blockIdx.x / 8;
+  if (wg >= nwg) return;
+  const int c0 = blockIdx.y * BT;
+  const int bw = min(BT, B - c0);
+  const int tid = threadIdx.x;
+  const int wave = tid / 64, lane = tid % 64;
+  const int lr = lane % 16, lk = lane / 16;
+  const int64_t e0 = wg * cpw, e1 = min((int64_t)g.ncell, e0 + cpw);
+  for (int64_t e = e0; e < e1; ++e) {
+    const int64_t cx = e % g.nc, cy = (e / g.nc) % g.nc, cz = e / ((int64_t)g.nc * g.nc);
+    if (e > e0) __syncthreads();  // previous cell's output read from xs
+    load_cell_input<T, TV, P, BT, LDB>(g, cx, cy, cz, S, X, xn, xcol, B, c0, bw, xs);
+    __syncthreads();
+    const vec_t* Hc = reinterpret_cast<const vec_t*>(H + (size_t)e * NTILE * 256) + lane;
+    acc_t acc[RT][NCT];
+#pragma unroll
+    for (int r = 0; r < RT; ++r)
+#pragma unroll
+      for (int ct = 0; ct < NCT; ++ct) acc[r][ct] = acc_t{0, 0, 0, 0};
+    auto run = [&](auto t0c, auto t1c) {
+      constexpr int T0 = decltype(t0c)::value, T1 = decltype(t1c)::value, CNT = T1 - T0;
+      vec_t buf[KPF];
+      static_for<0, (KPF < CNT ? KPF : CNT)>([&](auto jc) {
+        buf[decltype(jc)::value] = Hc[(T0 + decltype(jc)::value) * 64];
+      });
+      static_for<0, CNT>([&](auto jc) {
+        constexpr int J = decltype(jc)::value, t = T0 + J;
+        constexpr int r = tile_row(RT, t), k = tile_col(RT, t);
+        const vec_t a = buf[J % KPF];
+        if constexpr (J + KPF < CNT) buf[J % KPF] = Hc[(t + KPF) * 64];
+        {  // rows r += T (S X)_k
+          const T* xk = xs + (k * 16 + lk) * LDB + lr;
+          const T av[4] = {a.x, a.y, a.z, a.w};
+#pragma unroll
+          for (int s4 = 0; s4 < 4; ++s4)
+#pragma unroll
+            for (int ct = 0; ct < NCT; ++ct)
+              acc[r][ct] = Mfma<T>::run(av[s4], xk[s4 * 4 * LDB + ct * 16], acc[r][ct]);
+        }
+        if constexpr (r < k) {  // rows k += T^T (S X)_r
+          const vec_t at = lds_transpose<T>(a, tsq[wave], lane);
+          const T* xr = xs + (r * 16 + lk) * LDB + lr;
+          const T av[4] = {at.x, at.y, at.z, at.w};
+#pragma unroll
+          for (int s4 = 0; s4 < 4; ++s4)
+#pragma unroll
+            for (int ct = 0; ct < NCT; ++ct)
+              acc[k][ct] = Mfma<T>::run(av[s4], xr[s4 * 4 * LDB + ct * 16], acc[k][ct]);
+        }
+      });
+    };
+    constexpr int Q1 = NTILE / 4, Q2 = NTILE / 2, Q3 = (3 * NTILE) / 4;
+    switch (wave) {
+      case 0: run(ic<0>{}, ic<Q1>{}); break;
+      case 1: run(ic<Q1>{}, ic<Q2>{}); break;
+      case 2: run(ic<Q2>{}, ic<Q3>{}); break;
+      default: run(ic<Q3>{}, ic<NTILE>{}); break;
+    }
+    // partial sums of the four waves, added in wave order through xs
+    for (int w = 0; w < 4; ++w) {
+      __syncthreads();
+      if (wave == w) {
+#pragma unroll
+        for (int r = 0; r < RT; ++r)
+#pragma unroll
+          for (int ct = 0; ct < NCT; ++ct)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+              T* p = xs + (r * 16 + Mfma<T>::row(lk, i)) * LDB + ct * 16 + lr;
+              *p = w == 0 ? acc[r][ct][i] : *p + acc[r][ct][i];
+            }
+      }
+    }
+    __syncthreads();
+    T* yc = Yc + (size_t)e * NE * B + c0;
+    for (int i = tid; i < NE * BT; i += kThreads) {
+      const int l = i / BT, j = i % BT;
+      if (j < bw) yc[(size_t)l * B + j] = xs[l * LDB + j];
     }
   }
 }
@@ -1164,6 +1303,13 @@ static void cell_gemm_p(const Geo& g, const T* H, const T* S, const TV* X, const
   const int64_t nwg = (g.ncell + cpw - 1) / cpw;
   dim3 grid((unsigned)(8 * ((nwg + 7) / 8)), (unsigned)((B + 16 * nct - 1) / (16 * nct))),
       block(kThreads);
+  if (shape.tile_pass && nct <= (std::is_same<T, float>::value ? 2 : 1)) {
+    if (nct == 1) k_cell_symm<T, TV, P, 1><<<grid, block, 0, s>>>(g, H, S, X, xn, xcol, Yc, B, cpw);
+    if constexpr (std::is_same<T, float>::value) {
+      if (nct == 2) k_cell_symm<T, TV, P, 2><<<grid, block, 0, s>>>(g, H, S, X, xn, xcol, Yc, B, cpw);
+    }
+    return;
+  }
   switch (nct) {
     case 1: k_cell_gemm<T, TV, P, 1><<<grid, block, 0, s>>>(g, H, S, X, xn, xcol, Yc, B, cpw); break;
     case 2: k_cell_gemm<T, TV, P, 2><<<grid, block, 0, s>>>(g, H, S, X, xn, xcol, Yc, B, cpw); break;

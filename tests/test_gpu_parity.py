@@ -210,13 +210,14 @@ def test_every_tile_shape_keeps_parity(glib, olib, p, ncols):
     for cp in (1, 2, 4, 8):
         for vt in (16, 32, 64, 80):
             for prec, tol in ((abi.FP64, 1e-13), (abi.MIXED, 2e-6)):
-                g.set_apply_shape(prec, cp, vt)
-                assert rel(g.apply_hamiltonian(x, precision=prec), yo) < tol, (cp, vt, prec)
+                for tp in (0, 1):  # tile pass applies to 16 (FP64) / 32 (FP32) columns
+                    g.set_apply_shape(prec, cp, vt, tile_pass=tp)
+                    assert rel(g.apply_hamiltonian(x, precision=prec), yo) < tol, (cp, vt, tp, prec)
     for gr in (1, 2, 3, 7, 4096):
         for prec, tol in ((abi.FP64, 1e-13), (abi.MIXED, 2e-6)):
             g.set_apply_shape(prec, 1, 16, gr)
             assert rel(g.apply_hamiltonian(x, precision=prec), yo) < tol, (gr, prec)
     best = g.autotune_apply(ncols, abi.FP64, reps=2)
     assert best["cells_per_wg"] in (1, 2, 4, 8) and best["vectors_per_tile"] % 16 == 0
-    assert best["gather_rows"] in (0, 1, 2, 4, 8)
+    assert best["gather_rows"] in (0, 1, 2, 4, 8) and best["tile_pass"] in (0, 1)
     assert rel(g.apply_hamiltonian(x), yo) < 1e-13
