@@ -32,12 +32,12 @@ METRIC = "s per BFGS iter (eigen+adjoint+force), mixed vs FP64 on GPU, force err
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the dominant kernel from two rocprofv3 --pmc runs
 # (FETCH_SIZE, WRITE_SIZE) of `bench.py --kernels-only --shape 2 16 4 0` /
-# `... 2 16 0 1`, FETCH_SIZE scaled by the copy calibration (x1.926), WRITE_SIZE
+# `... 1 16 0 1`, FETCH_SIZE scaled by the copy calibration (x1.926), WRITE_SIZE
 # as read; the summary with the commands is
 # profiles/cell_gemm_inverse_24c_p5_pmc.md.
 MEASURED_TRAFFIC = {
     ("inverse_24c_p5", "k_cell_gemm<float,float,5,1>"): 2613e6,
-    ("inverse_24c_p5", "k_cell_symm<float,float,5,1>"): 1756e6,
+    ("inverse_24c_p5", "k_cell_symm<float,float,5,1>"): 1691e6,
 }
 
 

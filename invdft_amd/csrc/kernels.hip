@@ -165,14 +165,14 @@ __device__ inline typename Vec4<T>::type lds_transpose(typename Vec4<T>::type m,
 
 // The cell's input tile in LDS: xs[l][j] = S[n] xn[n] X[n][c0 + j] xcol[c0 + j]
 // (node n of local index l; rows l >= NE and columns j >= bw are zero).
-template <typename T, typename TV, int P, int BT, int LDB>
+template <typename T, typename TV, int P, int BT, int LDB, int NTH = kThreads>
 __device__ inline void load_cell_input(const Geo& g, int64_t cx, int64_t cy, int64_t cz,
                                        const T* __restrict__ S, const TV* __restrict__ X,
                                        const double* __restrict__ xn,
                                        const double* __restrict__ xcol, int B, int c0, int bw,
                                        T* xs) {
   constexpr int NE = ne_of(P), NEP = nep_of(P);
-  for (int i = threadIdx.x; i < NEP * BT; i += kThreads) {
+  for (int i = threadIdx.x; i < NEP * BT; i += NTH) {
     const int l = i / BT, j = i % BT;
     T v = T(0);
     if (l < NE && j < bw) {
@@ -313,26 +313,53 @@ __device__ inline void static_for(F&& f) {
   }
 }
 
-// Phase 1, tile pass: every stored tile of H_e is read from memory once.
-// Wave w owns the contiguous quarter [T0, T1) of the cell's NT stored tiles
-// (a compile-time list, so every accumulator index is static), adds
-// T (S X)_k to rows r and, for r < k, T^T (S X)_r (LDS transpose) to rows k,
-// keeping partial sums of all RT row blocks; the four waves' partials are
-// then added in LDS in wave order (deterministic) and written to Yc.  For
-// narrow column tiles (the accumulators hold RT x NCT MFMA tiles per wave).
+// Tiles of the stored upper triangle that touch block rows [r0, r1): the
+// j-th (in storage order) of those with r or k in the range, and their count.
+__host__ __device__ constexpr int half_tile_count(int rt_count, int r0, int r1) {
+  int n = 0;
+  for (int t = 0; t < rt_count * (rt_count + 1) / 2; ++t) {
+    const int r = tile_row(rt_count, t), k = tile_col(rt_count, t);
+    if ((r >= r0 && r < r1) || (k >= r0 && k < r1)) ++n;
+  }
+  return n;
+}
+__host__ __device__ constexpr int half_tile(int rt_count, int r0, int r1, int j) {
+  for (int t = 0; t < rt_count * (rt_count + 1) / 2; ++t) {
+    const int r = tile_row(rt_count, t), k = tile_col(rt_count, t);
+    if ((r >= r0 && r < r1) || (k >= r0 && k < r1)) {
+      if (j == 0) return t;
+      --j;
+    }
+  }
+  return -1;
+}
+
+constexpr int kSymmThreads = 512;
+
+// Phase 1, tile pass (8 waves per work-group).  The block rows are split in
+// two halves; waves 0-3 produce the rows of the first half, waves 4-7 those
+// of the second.  A half's four waves split, in compile-time lists (every
+// accumulator index static), the stored tiles (r, k) that touch the half:
+// T (S X)_k is added to row r if r is in the half and, for r < k,
+// T^T (S X)_r (LDS transpose) to row k if k is in the half.  A tile is read
+// once per half it touches -- the two halves read the shared ones at the same
+// time on the same CU, so the second read is a cache hit -- and each wave
+// keeps partial sums of its half's rows only.  The four partials of each half
+// are added in LDS in wave order (deterministic) and written to Yc.
 template <typename T, typename TV, int P, int NCT>
-__global__ __launch_bounds__(kThreads) void k_cell_symm(Geo g, const T* __restrict__ H,
-                                                        const T* __restrict__ S,
-                                                        const TV* __restrict__ X,
-                                                        const double* __restrict__ xn,
-                                                        const double* __restrict__ xcol,
-                                                        T* __restrict__ Yc, int B, int cpw) {
+__global__ __launch_bounds__(kSymmThreads) void k_cell_symm(Geo g, const T* __restrict__ H,
+                                                            const T* __restrict__ S,
+                                                            const TV* __restrict__ X,
+                                                            const double* __restrict__ xn,
+                                                            const double* __restrict__ xcol,
+                                                            T* __restrict__ Yc, int B, int cpw) {
   constexpr int NE = ne_of(P), NEP = nep_of(P), RT = NEP / 16, BT = NCT * 16;
   constexpr int NTILE = ntiles_of(P);
   constexpr int LDB = ldb_of(BT);
   constexpr int KPF = 3;
+  constexpr int RH = (RT + 1) / 2;  // block rows of the first half (and at most per half)
   __shared__ T xs[NEP * LDB];  // the input tile, then the partial-sum buffer
-  __shared__ T tsq[4][16 * 17];
+  __shared__ T tsq[8][16 * 17];
   typedef typename Vec4<T>::type vec_t;
   typedef typename Mfma<T>::acc_t acc_t;
   const int64_t nwg = (g.ncell + cpw - 1) / cpw;
@@ -348,35 +375,42 @@ __global__ __launch_bounds__(kThreads) void k_cell_symm(Geo g, const T* __restri
   for (int64_t e = e0; e < e1; ++e) {
     const int64_t cx = e % g.nc, cy = (e / g.nc) % g.nc, cz = e / ((int64_t)g.nc * g.nc);
     if (e > e0) __syncthreads();  // previous cell's output read from xs
-    load_cell_input<T, TV, P, BT, LDB>(g, cx, cy, cz, S, X, xn, xcol, B, c0, bw, xs);
+    load_cell_input<T, TV, P, BT, LDB, kSymmThreads>(g, cx, cy, cz, S, X, xn, xcol, B, c0, bw,
+                                                     xs);
     __syncthreads();
     const vec_t* Hc = reinterpret_cast<const vec_t*>(H + (size_t)e * NTILE * 256) + lane;
-    acc_t acc[RT][NCT];
+    acc_t acc[RH][NCT];
 #pragma unroll
-    for (int r = 0; r < RT; ++r)
+    for (int r = 0; r < RH; ++r)
 #pragma unroll
       for (int ct = 0; ct < NCT; ++ct) acc[r][ct] = acc_t{0, 0, 0, 0};
-    auto run = [&](auto t0c, auto t1c) {
-      constexpr int T0 = decltype(t0c)::value, T1 = decltype(t1c)::value, CNT = T1 - T0;
+    // tiles [J0, J1) of the list of half [R0, R1)
+    auto run = [&](auto r0c, auto r1c, auto j0c, auto j1c) {
+      constexpr int R0 = decltype(r0c)::value, R1 = decltype(r1c)::value;
+      constexpr int J0 = decltype(j0c)::value, CNT = decltype(j1c)::value - J0;
       vec_t buf[KPF];
       static_for<0, (KPF < CNT ? KPF : CNT)>([&](auto jc) {
-        buf[decltype(jc)::value] = Hc[(T0 + decltype(jc)::value) * 64];
+        constexpr int tp = half_tile(RT, R0, R1, J0 + decltype(jc)::value);
+        buf[decltype(jc)::value] = Hc[tp * 64];
       });
       static_for<0, CNT>([&](auto jc) {
-        constexpr int J = decltype(jc)::value, t = T0 + J;
+        constexpr int J = decltype(jc)::value, t = half_tile(RT, R0, R1, J0 + J);
         constexpr int r = tile_row(RT, t), k = tile_col(RT, t);
         const vec_t a = buf[J % KPF];
-        if constexpr (J + KPF < CNT) buf[J % KPF] = Hc[(t + KPF) * 64];
-        {  // rows r += T (S X)_k
+        if constexpr (J + KPF < CNT) {
+          constexpr int tn = half_tile(RT, R0, R1, J0 + J + KPF);
+          buf[J % KPF] = Hc[tn * 64];
+        }
+        if constexpr (r >= R0 && r < R1) {  // row r += T (S X)_k
           const T* xk = xs + (k * 16 + lk) * LDB + lr;
           const T av[4] = {a.x, a.y, a.z, a.w};
 #pragma unroll
           for (int s4 = 0; s4 < 4; ++s4)
 #pragma unroll
             for (int ct = 0; ct < NCT; ++ct)
-              acc[r][ct] = Mfma<T>::run(av[s4], xk[s4 * 4 * LDB + ct * 16], acc[r][ct]);
+              acc[r - R0][ct] = Mfma<T>::run(av[s4], xk[s4 * 4 * LDB + ct * 16], acc[r - R0][ct]);
         }
-        if constexpr (r < k) {  // rows k += T^T (S X)_r
+        if constexpr (r < k && k >= R0 && k < R1) {  // row k += T^T (S X)_r
           const vec_t at = lds_transpose<T>(a, tsq[wave], lane);
           const T* xr = xs + (r * 16 + lk) * LDB + lr;
           const T av[4] = {at.x, at.y, at.z, at.w};
@@ -384,35 +418,43 @@ __global__ __launch_bounds__(kThreads) void k_cell_symm(Geo g, const T* __restri
           for (int s4 = 0; s4 < 4; ++s4)
 #pragma unroll
             for (int ct = 0; ct < NCT; ++ct)
-              acc[k][ct] = Mfma<T>::run(av[s4], xr[s4 * 4 * LDB + ct * 16], acc[k][ct]);
+              acc[k - R0][ct] = Mfma<T>::run(av[s4], xr[s4 * 4 * LDB + ct * 16], acc[k - R0][ct]);
         }
       });
     };
-    constexpr int Q1 = NTILE / 4, Q2 = NTILE / 2, Q3 = (3 * NTILE) / 4;
+    constexpr int C0 = half_tile_count(RT, 0, RH), C1 = half_tile_count(RT, RH, RT);
     switch (wave) {
-      case 0: run(ic<0>{}, ic<Q1>{}); break;
-      case 1: run(ic<Q1>{}, ic<Q2>{}); break;
-      case 2: run(ic<Q2>{}, ic<Q3>{}); break;
-      default: run(ic<Q3>{}, ic<NTILE>{}); break;
+      case 0: run(ic<0>{}, ic<RH>{}, ic<0>{}, ic<C0 / 4>{}); break;
+      case 1: run(ic<0>{}, ic<RH>{}, ic<C0 / 4>{}, ic<C0 / 2>{}); break;
+      case 2: run(ic<0>{}, ic<RH>{}, ic<C0 / 2>{}, ic<(3 * C0) / 4>{}); break;
+      case 3: run(ic<0>{}, ic<RH>{}, ic<(3 * C0) / 4>{}, ic<C0>{}); break;
+      case 4: run(ic<RH>{}, ic<RT>{}, ic<0>{}, ic<C1 / 4>{}); break;
+      case 5: run(ic<RH>{}, ic<RT>{}, ic<C1 / 4>{}, ic<C1 / 2>{}); break;
+      case 6: run(ic<RH>{}, ic<RT>{}, ic<C1 / 2>{}, ic<(3 * C1) / 4>{}); break;
+      default: run(ic<RH>{}, ic<RT>{}, ic<(3 * C1) / 4>{}, ic<C1>{}); break;
     }
-    // partial sums of the four waves, added in wave order through xs
+    // partial sums of each half's four waves, added in wave order through xs
+    // (the halves own disjoint rows, so they reduce side by side)
+    const int r0 = wave < 4 ? 0 : RH, nr = wave < 4 ? RH : RT - RH;
     for (int w = 0; w < 4; ++w) {
       __syncthreads();
-      if (wave == w) {
+      if (wave % 4 == w) {
 #pragma unroll
-        for (int r = 0; r < RT; ++r)
+        for (int r = 0; r < RH; ++r) {
+          if (r >= nr) break;
 #pragma unroll
           for (int ct = 0; ct < NCT; ++ct)
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-              T* p = xs + (r * 16 + Mfma<T>::row(lk, i)) * LDB + ct * 16 + lr;
+              T* p = xs + ((r0 + r) * 16 + Mfma<T>::row(lk, i)) * LDB + ct * 16 + lr;
               *p = w == 0 ? acc[r][ct][i] : *p + acc[r][ct][i];
             }
+        }
       }
     }
     __syncthreads();
     T* yc = Yc + (size_t)e * NE * B + c0;
-    for (int i = tid; i < NE * BT; i += kThreads) {
+    for (int i = tid; i < NE * BT; i += kSymmThreads) {
       const int l = i / BT, j = i % BT;
       if (j < bw) yc[(size_t)l * B + j] = xs[l * LDB + j];
     }
@@ -1304,9 +1346,11 @@ static void cell_gemm_p(const Geo& g, const T* H, const T* S, const TV* X, const
   dim3 grid((unsigned)(8 * ((nwg + 7) / 8)), (unsigned)((B + 16 * nct - 1) / (16 * nct))),
       block(kThreads);
   if (shape.tile_pass && nct <= (std::is_same<T, float>::value ? 2 : 1)) {
-    if (nct == 1) k_cell_symm<T, TV, P, 1><<<grid, block, 0, s>>>(g, H, S, X, xn, xcol, Yc, B, cpw);
+    const dim3 block2(kSymmThreads);
+    if (nct == 1) k_cell_symm<T, TV, P, 1><<<grid, block2, 0, s>>>(g, H, S, X, xn, xcol, Yc, B, cpw);
     if constexpr (std::is_same<T, float>::value) {
-      if (nct == 2) k_cell_symm<T, TV, P, 2><<<grid, block, 0, s>>>(g, H, S, X, xn, xcol, Yc, B, cpw);
+      if (nct == 2)
+        k_cell_symm<T, TV, P, 2><<<grid, block2, 0, s>>>(g, H, S, X, xn, xcol, Yc, B, cpw);
     }
     return;
   }
