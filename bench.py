@@ -196,9 +196,18 @@ def uneven_batch(lib, count, steps, workers):
         probs += batch.build_problems(lib, params[k:k + 8])
         log(f"  set-up {len(probs)}/{count} problems ({time.perf_counter() - t0:.1f} s)")
     tol_kw = dict(cheb_tol=1e-8, initial_tol_cheb=1e-8)
-    res_m, wall_m = batch.run_concurrent(lib, probs, abi.MIXED, steps, workers, tol_kw)
+    done = [0]
+
+    def progress(k, r):
+        done[0] += 1
+        p = params[k]
+        log(f"    problem {k} ({p['n_cells']}^3 cells, p={p['degree']}, {p['n_eig']} states): "
+            f"{r[0]:.2f} s, loss {r[1]:.3e} -> {r[2]:.3e} [{done[0]}/{count}]")
+
+    res_m, wall_m = batch.run_concurrent(lib, probs, abi.MIXED, steps, workers, tol_kw, progress)
     log(f"  mixed, {workers} concurrent contexts: {wall_m:.1f} s")
-    res_f, wall_f = batch.run_sequential(lib, probs, abi.FP64, steps, tol_kw)
+    done[0] = 0
+    res_f, wall_f = batch.run_sequential(lib, probs, abi.FP64, steps, tol_kw, progress)
     log(f"  FP64 sequential: {wall_f:.1f} s")
     sm, sf = batch.summary(res_m, wall_m), batch.summary(res_f, wall_f)
     out = {"metric": f"problems/s, uneven batch of {count} inverse problems x {steps} BFGS steps",

@@ -87,7 +87,7 @@ typedef struct idft_desc {
   double rho_tol_constraints; /* rhoTolForConstraints (.cpp:413)                */
   int32_t gradient_mode;      /* IDFT_GRADIENT_*                                */
   int32_t lanczos_steps;      /* steps of the spectrum upper-bound Lanczos      */
-  int32_t refine_every;       /* MIXED: inner MINRES steps per FP64 refinement  */
+  int32_t refine_every;       /* reserved (no effect; keeps the layout)         */
   int32_t check_every;        /* MINRES steps between host convergence checks   */
   uint64_t seed;              /* seed of the initial random subspace            */
   int32_t minres_graph;       /* HIP: replay check_every MINRES steps as one

@@ -55,8 +55,9 @@ def _solve(lib, prob, precision, steps, tol_kw):
     return dt, float(loss[0]), float(loss[-1]), n
 
 
-def run_concurrent(lib, probs, precision, steps, workers=4, tol_kw=None):
-    """Per-problem seconds and the wall time of the whole batch."""
+def run_concurrent(lib, probs, precision, steps, workers=4, tol_kw=None, progress=None):
+    """Per-problem seconds and the wall time of the whole batch; progress(k,
+    result) is called as each problem finishes."""
     tol_kw = tol_kw or {}
     res = [None] * len(probs)
     nxt = [0]
@@ -72,6 +73,9 @@ def run_concurrent(lib, probs, precision, steps, workers=4, tol_kw=None):
                 return
             try:
                 res[k] = _solve(lib, probs[k], precision, steps, tol_kw)
+                if progress:
+                    with lock:
+                        progress(k, res[k])
             except Exception as e:  # surfaced after the join
                 errors.append((k, e))
                 return
@@ -89,8 +93,9 @@ def run_concurrent(lib, probs, precision, steps, workers=4, tol_kw=None):
     return res, wall
 
 
-def run_sequential(lib, probs, precision, steps, tol_kw=None):
-    return run_concurrent(lib, probs, precision, steps, workers=1, tol_kw=tol_kw)
+def run_sequential(lib, probs, precision, steps, tol_kw=None, progress=None):
+    return run_concurrent(lib, probs, precision, steps, workers=1, tol_kw=tol_kw,
+                          progress=progress)
 
 
 def summary(res, wall):

@@ -137,7 +137,8 @@ struct ApplyEpilogue {
   double* dot_partial = nullptr;
 };
 constexpr int kGatherBlocks = 16384;
-int gather_blocks(const Geo& g, int B, int rows);  // rows of the gather's dot partials
+// rows of the gather's dot partials for vectors of vbytes-byte elements
+int gather_blocks(const Geo& g, int B, int rows, int vbytes);
 // Cell operator in two launches: per-cell H_e (S xcol Xin)_e into the scratch
 // Yc (cell_scratch_elems(g, B) elements), then the node gather + epilogue.
 // Returns the number of gather blocks (rows of dot_partial).

@@ -649,8 +649,8 @@ struct idft_ctx {
     launch_lincomb<double>(N, b, W.w1.p, 0.0, nullptr, 0.0, nullptr, nullptr, 0.0, nullptr, nullptr, s);
     launch_lincomb<double>(N, b, W.w.p, 0.0, nullptr, 0.0, nullptr, nullptr, 0.0, nullptr, nullptr, s);
     const bool canRelax = relax > 0.0 && H32.p != nullptr;
-    int nblk = std::max(1024, gather_blocks(g, b, shape64.gather_rows));
-    if (canRelax) nblk = std::max(nblk, gather_blocks(g, b, shape32.gather_rows));
+    int nblk = std::max(1024, gather_blocks(g, b, shape64.gather_rows, 8));
+    if (canRelax) nblk = std::max(nblk, gather_blocks(g, b, shape32.gather_rows, 8));
     partial.resize((size_t)nblk * b);
     int nb = launch_coldot_partial<double>(N, b, W.r1.p, W.r1.p, Djac.p, partial.p, s);
     launch_minres_start(b, nb, partial.p, adj.dnegEig.p, mst.p, s);

@@ -471,7 +471,34 @@ __global__ __launch_bounds__(kSymmThreads) void k_cell_symm(Geo g, const T* __re
 // (iy, iz) in [by * R, by * R + R) and x positions [bx * per, bx * per + per)
 // of each row; thread t owns column t % B and x offset t / B for its whole
 // life.
-template <typename TY, typename T, int P>
+// VW consecutive elements of a node-major row (16-byte aligned for VW > 1).
+template <typename T, int VW>
+__device__ inline void vload(const T* __restrict__ p, T (&o)[VW]) {
+  if constexpr (VW == 1) {
+    o[0] = p[0];
+  } else if constexpr (VW == 2) {
+    typedef typename std::conditional<std::is_same<T, float>::value, float2, double2>::type v2;
+    const v2 v = *reinterpret_cast<const v2*>(p);
+    o[0] = v.x; o[1] = v.y;
+  } else {
+    const typename Vec4<T>::type v = *reinterpret_cast<const typename Vec4<T>::type*>(p);
+    o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
+  }
+}
+template <typename T, int VW>
+__device__ inline void vstore(T* __restrict__ p, const T (&o)[VW]) {
+  if constexpr (VW == 1) {
+    p[0] = o[0];
+  } else if constexpr (VW == 2) {
+    typedef typename std::conditional<std::is_same<T, float>::value, float2, double2>::type v2;
+    *reinterpret_cast<v2*>(p) = v2{o[0], o[1]};
+  } else {
+    typedef typename Vec4<T>::type v4;
+    *reinterpret_cast<v4*>(p) = v4{o[0], o[1], o[2], o[3]};
+  }
+}
+
+template <typename TY, typename T, int P, int VW>
 __global__ __launch_bounds__(kThreads) void k_cell_gather(
     Geo g, uint32_t B, uint32_t R, const TY* __restrict__ Yc, const T* __restrict__ S,
     T* __restrict__ Y, double alpha, double beta, const T* __restrict__ X, double a,
@@ -480,17 +507,24 @@ __global__ __launch_bounds__(kThreads) void k_cell_gather(
     const double* __restrict__ cc, const T* __restrict__ Xin, const double* __restrict__ xn,
     const double* __restrict__ xcol, T* __restrict__ V, double* __restrict__ partial) {
   constexpr uint32_t n1 = P + 1, ne = n1 * n1 * n1;
-  __shared__ double red[kThreads];
+  __shared__ double red[kThreads * VW];
   const uint32_t N1 = g.N1, nc = g.nc;
-  const uint32_t tid = threadIdx.x, per = kThreads / B;
-  const uint32_t j = tid % B, off = tid / B;
+  const uint32_t nb = B / VW;  // column groups of a node
+  const uint32_t tid = threadIdx.x, per = kThreads / nb;
+  const uint32_t jg = tid % nb, off = tid / nb, j0 = jg * VW;
   const uint32_t ix = blockIdx.x * per + off;
-  double dot = 0.0;
+  double dot[VW];
+#pragma unroll
+  for (int v = 0; v < VW; ++v) dot[v] = 0.0;
   if (off < per && ix < N1) {
-    const double caj = X ? a * (ca ? ca[j] : 1.0) : 0.0;
-    const double cbj = Z ? b * (cb ? cb[j] : 1.0) : 0.0;
-    const double ccj = W ? c * (cc ? cc[j] : 1.0) : 0.0;
-    const double xs = xcol ? xcol[j] : 1.0;
+    double caj[VW], cbj[VW], ccj[VW], xs[VW];
+#pragma unroll
+    for (int v = 0; v < VW; ++v) {
+      caj[v] = X ? a * (ca ? ca[j0 + v] : 1.0) : 0.0;
+      cbj[v] = Z ? b * (cb ? cb[j0 + v] : 1.0) : 0.0;
+      ccj[v] = W ? c * (cc ? cc[j0 + v] : 1.0) : 0.0;
+      xs[v] = xcol ? xcol[j0 + v] : 1.0;
+    }
     // x contributions (cell, local index) of this thread's column of nodes
     uint32_t cxs[2], axs[2];
     int nx = 0;
@@ -522,46 +556,67 @@ __global__ __launch_bounds__(kThreads) void k_cell_gather(
       }
       // all (up to 8) contributions are loaded before the first is summed:
       // predicated, fully unrolled, so the loads are in flight together
-      TY c8[8];
+      TY c8[8][VW];
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
         const int kz = k >> 2, ky = (k >> 1) & 1, kx = k & 1;
-        c8[k] = TY(0);
+#pragma unroll
+        for (int v = 0; v < VW; ++v) c8[k][v] = TY(0);
         if (kz < nz && ky < ny && kx < nx) {
           const uint64_t e = ((uint64_t)czs[kz] * nc + cys[ky]) * nc + cxs[kx];
           const uint32_t l = (azs[kz] * n1 + ays[ky]) * n1 + axs[kx];
-          c8[k] = Yc[(e * ne + l) * B + j];
+          vload<TY, VW>(Yc + (e * ne + l) * B + j0, c8[k]);
         }
       }
-      double sum = 0.0;
+      double sum[VW];
+#pragma unroll
+      for (int v = 0; v < VW; ++v) sum[v] = 0.0;
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
         const int kz = k >> 2, ky = (k >> 1) & 1, kx = k & 1;
-        if (kz < nz && ky < ny && kx < nx) sum += (double)c8[k];
+        if (kz < nz && ky < ny && kx < nx)
+#pragma unroll
+          for (int v = 0; v < VW; ++v) sum[v] += (double)c8[k][v];
       }
       const uint32_t n = row * N1 + ix;
-      const uint32_t i = n * B + j;
+      const uint32_t i = n * B + j0;
       const double sn = xn ? xn[n] : 1.0;
-      double v = alpha * (double)S[n] * sum;
-      if (X) v += caj * (sn * (double)X[i]);
-      if (Z) v += cbj * (double)Z[i];
-      if (W) v += ccj * (double)W[i];
-      if (beta != 0.0) v += beta * (double)Y[i];
-      const T yo = (T)v;
-      if (partial) {
-        const T vv = (T)(xs * (sn * (double)Xin[i]));
-        if (V) V[i] = vv;
-        dot += (double)vv * (double)yo;
+      const double sa = alpha * (double)S[n];
+      T xv[VW], zv[VW], wv[VW], yv[VW], xin[VW], yo[VW];
+      if (X) vload<T, VW>(X + i, xv);
+      if (Z) vload<T, VW>(Z + i, zv);
+      if (W) vload<T, VW>(W + i, wv);
+      if (beta != 0.0) vload<T, VW>(Y + i, yv);
+      if (partial) vload<T, VW>(Xin + i, xin);
+#pragma unroll
+      for (int v = 0; v < VW; ++v) {
+        double t = sa * sum[v];
+        if (X) t += caj[v] * (sn * (double)xv[v]);
+        if (Z) t += cbj[v] * (double)zv[v];
+        if (W) t += ccj[v] * (double)wv[v];
+        if (beta != 0.0) t += beta * (double)yv[v];
+        yo[v] = (T)t;
       }
-      Y[i] = yo;
+      if (partial) {
+        T vv[VW];
+#pragma unroll
+        for (int v = 0; v < VW; ++v) {
+          vv[v] = (T)(xs[v] * (sn * (double)xin[v]));
+          dot[v] += (double)vv[v] * (double)yo[v];
+        }
+        if (V) vstore<T, VW>(V + i, vv);
+      }
+      vstore<T, VW>(Y + i, yo);
     }
   }
   if (partial) {
-    red[tid] = dot;
+#pragma unroll
+    for (int v = 0; v < VW; ++v) red[tid * VW + v] = dot[v];
     __syncthreads();
     if (tid < B) {
+      const uint32_t g2 = tid / VW, v = tid % VW;
       double t = 0.0;
-      for (uint32_t k = 0; k < per; ++k) t += red[k * B + tid];
+      for (uint32_t k = 0; k < per; ++k) t += red[(k * nb + g2) * VW + v];
       partial[((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * B + tid] = t;
     }
   }
@@ -1365,12 +1420,20 @@ static void cell_gemm_p(const Geo& g, const T* H, const T* S, const TV* X, const
 
 int64_t cell_scratch_elems(const Geo& g, int B) { return g.ncell * (int64_t)ne_of(g.p) * B; }
 
-// Gather grid: x chunks of kThreads / B nodes times groups of R node rows,
-// with R = `rows_req` if positive, else chosen so that the grid has at most
-// about kGatherBlocks blocks.
-static void gather_grid(const Geo& g, int B, int rows_req, unsigned& gx, unsigned& gy,
+// Columns per gather thread: 16-byte vectors of the node-major rows where B
+// allows it.
+static int gather_vw(int B, int vbytes) {
+  const int w = 16 / vbytes;
+  if (B % w == 0) return w;
+  return B % 2 == 0 ? 2 : 1;
+}
+
+// Gather grid: x chunks of kThreads / (B / VW) nodes times groups of R node
+// rows, with R = `rows_req` if positive, else chosen so that the grid has at
+// most about kGatherBlocks blocks.
+static void gather_grid(const Geo& g, int B, int vw, int rows_req, unsigned& gx, unsigned& gy,
                         uint32_t& R) {
-  const int per = kThreads / B;
+  const int per = kThreads / (B / vw);
   gx = (unsigned)((g.N1 + per - 1) / per);
   const int64_t rows = (int64_t)g.N1 * g.N1;
   int64_t r = rows_req > 0 ? rows_req : (rows * gx + kGatherBlocks - 1) / kGatherBlocks;
@@ -1379,10 +1442,10 @@ static void gather_grid(const Geo& g, int B, int rows_req, unsigned& gx, unsigne
   gy = (unsigned)((rows + r - 1) / r);
 }
 
-int gather_blocks(const Geo& g, int B, int rows) {
+int gather_blocks(const Geo& g, int B, int rows, int vbytes) {
   unsigned gx, gy;
   uint32_t R;
-  gather_grid(g, B, rows, gx, gy, R);
+  gather_grid(g, B, gather_vw(B, vbytes), rows, gx, gy, R);
   return (int)(gx * gy);
 }
 
@@ -1419,19 +1482,24 @@ int launch_cell_apply(const Geo& g, const T* H, const T* Sg, const TV* S, const 
   check_u32(g.N * B);
   unsigned gx, gy;
   uint32_t R;
-  gather_grid(g, B, shape.gather_rows, gx, gy, R);
+  const int vw = gather_vw(B, (int)sizeof(TV));
+  gather_grid(g, B, vw, shape.gather_rows, gx, gy, R);
   dim3 grid(gx, gy);
-#define IDFT_GATHER(PP)                                                                       \
-  k_cell_gather<T, TV, PP><<<grid, kThreads, 0, s>>>(g, (uint32_t)B, R, Yc, S, ep.Y, ep.alpha, \
-                                                 ep.beta, ep.X, ep.a, ep.ca, ep.Z, ep.b, ep.cb, \
-                                                 ep.W, ep.c, ep.cc, X, ep.xn, ep.xcol, ep.V,  \
-                                                 ep.dot_partial)
+#define IDFT_GATHER(PP, VW)                                                                  \
+  k_cell_gather<T, TV, PP, VW><<<grid, kThreads, 0, s>>>(                                    \
+      g, (uint32_t)B, R, Yc, S, ep.Y, ep.alpha, ep.beta, ep.X, ep.a, ep.ca, ep.Z, ep.b, ep.cb, \
+      ep.W, ep.c, ep.cc, X, ep.xn, ep.xcol, ep.V, ep.dot_partial)
+#define IDFT_GATHER_P(PP)                                         \
+  if (vw == 4) { if constexpr (sizeof(TV) == 4) IDFT_GATHER(PP, 4); } \
+  else if (vw == 2) IDFT_GATHER(PP, 2);                           \
+  else IDFT_GATHER(PP, 1);
   switch (g.p) {
-    case 2: IDFT_GATHER(2); break;
-    case 3: IDFT_GATHER(3); break;
-    case 4: IDFT_GATHER(4); break;
-    case 5: IDFT_GATHER(5); break;
+    case 2: IDFT_GATHER_P(2); break;
+    case 3: IDFT_GATHER_P(3); break;
+    case 4: IDFT_GATHER_P(4); break;
+    case 5: IDFT_GATHER_P(5); break;
   }
+#undef IDFT_GATHER_P
 #undef IDFT_GATHER
   IDFT_HIP(hipGetLastError());
   return (int)(gx * gy);
