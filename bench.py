@@ -30,6 +30,20 @@ sys.path.insert(0, HERE)
 
 METRIC = "s per BFGS iter (eigen+adjoint+force), mixed vs FP64 on GPU, force err ≤1e-6"
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3  # MI355X published FP32 vector / dense matrix peak
+
+
+def sf_flops_per_cell_column(p):
+    """Flops of the sum-factorised cell product per cell and column: the 1D
+    contractions of its five stages (2 per multiply-add), the quadrature
+    weight multiply and the kinetic scale."""
+    n1, q = p + 1, p + 3
+    fma = (n1 ** 3 * (q + 2 * n1)            # x lines: Lq, M, D
+           + n1 ** 2 * q ** 2 + 3 * n1 ** 4  # y lines: Lq; M, D + M
+           + 2 * q ** 3 * n1 + 2 * n1 ** 4   # z lines: Lq and Lq^T; D + M
+           + n1 ** 2 * q ** 2                # y lines back
+           + n1 ** 3 * q)                    # x lines back
+    return 2 * fma + q ** 3 + n1 ** 3
 # HBM bytes per launch of the dominant kernel from two rocprofv3 --pmc runs
 # (FETCH_SIZE, WRITE_SIZE) of `bench.py --kernels-only --shape 2 16 4 0` /
 # `... 1 16 0 1`, FETCH_SIZE scaled by the copy calibration (x1.926), WRITE_SIZE
@@ -296,38 +310,59 @@ def main():
         force_err = float(np.linalg.norm(fm - ff) / np.linalg.norm(ff))
         log(f"force rel err mixed vs FP64 (tight solves): {force_err:.3e}; loss {lm:.8e} vs {lf:.8e}")
 
-    # roofline of the dominant kernel: the FP32 MFMA cell-product kernel
-    # k_cell_gemm (phase 1 of the operator).  Algorithmic bytes per launch: the
-    # symmetric H_e (ne(ne+1)/2 entries per cell) read once, the block X and
-    # the node scaling S read once, and the per-cell products written once
-    # (the gather kernel reads them back).
+    # roofline of the dominant kernel, the FP32 cell product (phase 1 of the
+    # operator), for the kernel the autotuner chose:
+    #  - stored-H kernels (k_cell_gemm, k_cell_symm) are HBM-bound; their
+    #    algorithmic bytes per launch are the symmetric H_e (ne(ne+1)/2 entries
+    #    per cell) read once, the block X and node scale S read once and the
+    #    per-cell products written once (the gather kernel reads them back);
+    #  - the sum-factorised kernel (k_cell_sf) is bound by the vector ALUs; its
+    #    algorithmic work per launch is the 1D contractions, sf_flops(p) per
+    #    cell and column, and it moves w_e + X + S + the per-cell products.
     s = cm.sizes
     B = prob.desc.n_eig
+    p = prob.desc.degree
     ne = s.nodes_per_cell
     nh = ne * (ne + 1) // 2
+    nq = s.quad_per_cell
 
     def gemm_bytes(sz):
         return s.n_cells * nh * sz + s.n_nodes * (B + 1) * sz + s.n_cells * ne * B * sz
 
+    def sf_bytes(sz):
+        return s.n_cells * nq * sz + s.n_nodes * (B + 1) * sz + s.n_cells * ne * B * sz
+
+    def sf_flops():
+        return s.n_cells * B * sf_flops_per_cell_column(p)
+
     ms_op, ms_gemm = cm.profile_apply(B, abi.MIXED, reps=20)
-    bytes_gemm = gemm_bytes(4)
-    bytes_op = s.n_cells * nh * 4 + 2 * s.n_nodes * B * 4
-    achieved = bytes_gemm / (ms_gemm * 1e-3) / 1e9
     tuned32 = mixed["tuned"]["fp32"]
-    nct = min(5, max(1, min(tuned32["vectors_per_tile"], (B + 15) // 16 * 16) // 16))
-    kname = (f"k_cell_{'symm' if tuned32['tile_pass'] else 'gemm'}<float,float,"
-             f"{prob.desc.degree},{nct}>")
-    traffic = MEASURED_TRAFFIC.get((args.config, kname))
-    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "kernel": kname, "ms_per_launch": round(ms_gemm, 4), "bytes_per_launch": bytes_gemm,
-            "operator_ms": round(ms_op, 4),
-            "operator_GBs": round(bytes_op / (ms_op * 1e-3) / 1e9, 1)}
+    kern = tuned32["kernel"]
+    if kern == abi.KERNEL_SUM_FACTOR:
+        kname = (f"k_cell_sf<float,float,{p},{tuned32['vectors_per_tile']},"
+                 f"{tuned32['cells_per_wg']}>")
+        fl = sf_flops()
+        tf = fl / (ms_gemm * 1e-3) / 1e12
+        roof = {"bound": "valu", "achieved": round(tf, 2), "peak": FP32_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(tf / FP32_PEAK_TFLOPS, 4),
+                "traffic": MEASURED_TRAFFIC.get((args.config, kname)), "kernel": kname,
+                "ms_per_launch": round(ms_gemm, 4), "flops_per_launch": fl,
+                "bytes_per_launch": sf_bytes(4),
+                "achieved_GBs": round(sf_bytes(4) / (ms_gemm * 1e-3) / 1e9, 1)}
+    else:
+        nct = min(5, max(1, min(tuned32["vectors_per_tile"], (B + 15) // 16 * 16) // 16))
+        kname = f"k_cell_{('gemm', 'symm')[kern]}<float,float,{p},{nct}>"
+        achieved = gemm_bytes(4) / (ms_gemm * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": MEASURED_TRAFFIC.get((args.config, kname)), "kernel": kname,
+                "ms_per_launch": round(ms_gemm, 4), "bytes_per_launch": gemm_bytes(4)}
+    roof["operator_ms"] = round(ms_op, 4)
     extra = {}
     if cf is not None:
         m64, g64 = cf.profile_apply(B, abi.FP64, reps=20)
-        extra["fp64_gemm"] = {"ms_per_launch": round(g64, 4), "operator_ms": round(m64, 4),
-                              "achieved_GBs": round(gemm_bytes(8) / (g64 * 1e-3) / 1e9, 1)}
+        extra["fp64_cell_product"] = {"ms_per_launch": round(g64, 4), "operator_ms": round(m64, 4),
+                                      "shape": fp64["tuned"]["fp64"]}
     cpu = None
     if not args.no_cpu:
         log("CPU baseline sample (oracle)")
@@ -344,7 +379,7 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32-mfma+f64",
+        "dtype": "f32+f64",
         "data": "synthetic",
         "config": {"workload": args.config, "n_cells": cfg["n_cells"], "degree": cfg["degree"],
                    "nodes": int(s.n_nodes), "orbitals": B, "cheb_degree": prob.desc.cheb_degree,

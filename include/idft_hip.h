@@ -26,27 +26,39 @@ int idft_hip_init(void);
 int idft_profile_apply(idft_ctx* ctx, int32_t ncols, int32_t precision, int32_t reps,
                        double* ms_per_apply, double* ms_gemm);
 
-/* Sets the operator launch shape used at the given precision: the
- * cell-product tile (cells_per_wg in 1..64, vectors_per_tile a multiple of
- * 16 in 16..80), tile_pass (1: the cell product reads every stored tile of
- * H_e once and keeps partial sums of all block rows per wave -- used for
- * 16-column tiles, and 32 in FP32; 0: one wave per block row, mirror tiles
- * read again) and the node rows per node-gather work-group (gather_rows in
- * 1..4096, or 0 for the default of about 16384 gather work-groups). */
-int idft_set_apply_shape(idft_ctx* ctx, int32_t precision, int32_t cells_per_wg,
-                         int32_t vectors_per_tile, int32_t tile_pass, int32_t gather_rows);
+/* Cell-product kernels (the `kernel` argument below). */
+#define IDFT_KERNEL_ROW_PASS 0   /* stored H_e tiles, one wave per 16-row block */
+#define IDFT_KERNEL_TILE_PASS 1  /* stored H_e tiles, each read once per row half */
+#define IDFT_KERNEL_SUM_FACTOR 2 /* no stored H_e: 1D contractions with the cell's
+                                    quadrature weights (sum factorisation) */
 
-/* Autotuner of the cell operator: times every tile shape
+/* Sets the operator launch shape used at the given precision: the
+ * cell-product kernel and its shape, and the node rows per node-gather
+ * work-group (gather_rows in 1..4096, or 0 for the default of about 16384
+ * gather work-groups).
+ *   IDFT_KERNEL_ROW_PASS / IDFT_KERNEL_TILE_PASS: cells_per_wg in 1..64,
+ *     vectors_per_tile a multiple of 16 in 16..80; the tile pass keeps
+ *     partial sums of all block rows per wave and applies to 16-column tiles
+ *     (and 32 in FP32), wider tiles run the row pass.  The stored H_e is
+ *     built on first use.
+ *   IDFT_KERNEL_SUM_FACTOR: vectors_per_tile = columns per work-group, 2, 4
+ *     or 8 (FP64) and also 16 (FP32); cells_per_wg in 1..64, unused. */
+int idft_set_apply_shape(idft_ctx* ctx, int32_t precision, int32_t cells_per_wg,
+                         int32_t vectors_per_tile, int32_t kernel, int32_t gather_rows);
+
+/* Autotuner of the cell operator: times the sum-factorised kernel at every
+ * supported column count per work-group, then every stored-H shape
  * cells-per-work-group in {1,2,4,8} x vectors-per-tile in {16,32,64} (tiles
- * wider than the block collapse to one pass) x tile_pass {0,1} where it
- * applies, then node rows per gather work-group in {default,1,2,4,8} with
- * the best tile, with idft_profile_apply on ncols columns at the given
- * precision, keeps the fastest for this context and precision, and reports
- * it.  The two cell-product kernels sum in different orders, so results
- * agree to rounding, not bit for bit, across tile_pass.  No counterpart in
- * the reference (its cell matrices go through a library batched GEMM). */
+ * wider than the block collapse to one pass) x {row pass, tile pass} where
+ * it applies, then node rows per gather work-group in {default,1,2,4,8} with
+ * the best, with idft_profile_apply on ncols columns at the given precision,
+ * keeps the fastest for this context and precision, reports it, and frees
+ * the stored H_e when no precision's kernel reads it.  The kernels sum in
+ * different orders, so results agree to rounding, not bit for bit, across
+ * kernels.  No counterpart in the reference (its cell matrices go through a
+ * library batched GEMM). */
 int idft_autotune_apply(idft_ctx* ctx, int32_t ncols, int32_t precision, int32_t reps,
-                        int32_t* cells_per_wg, int32_t* vectors_per_tile, int32_t* tile_pass,
+                        int32_t* cells_per_wg, int32_t* vectors_per_tile, int32_t* kernel,
                         int32_t* gather_rows, double* ms_best);
 
 #ifdef __cplusplus

@@ -13,6 +13,8 @@ IDFT_OK, IDFT_EINVAL, IDFT_EHIP, IDFT_ENOCONV, IDFT_ENAN, IDFT_ESTATE, IDFT_ENOM
 FP64, MIXED = 0, 1
 GRADIENT_REFERENCE, GRADIENT_EXACT = 0, 1
 LS_SECANT_LOSS, LS_SECANT_FORCE_NORM, LS_CP = 0, 1, 2
+# cell-product kernels (include/idft_hip.h)
+KERNEL_ROW_PASS, KERNEL_TILE_PASS, KERNEL_SUM_FACTOR = 0, 1, 2
 
 _DP = C.POINTER(C.c_double)
 _IP = C.POINTER(C.c_int32)
@@ -443,12 +445,14 @@ class Context:
         return lh[: nd.value + 1], gh[: nd.value + 1], nd.value
 
     def set_apply_shape(self, precision, cells_per_wg, vectors_per_tile, gather_rows=0,
-                        tile_pass=0):
+                        kernel=KERNEL_ROW_PASS):
+        """Cell-product kernel and launch shape at one precision (HIP library
+        only); for KERNEL_SUM_FACTOR vectors_per_tile is the number of columns
+        per work-group (2, 4, 8, and 16 in FP32)."""
         fn = self.lib.idft_set_apply_shape
         fn.restype = C.c_int
         fn.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32]
-        self._check(fn(self.h, precision, cells_per_wg, vectors_per_tile, tile_pass,
-                       gather_rows))
+        self._check(fn(self.h, precision, cells_per_wg, vectors_per_tile, kernel, gather_rows))
 
     def autotune_apply(self, ncols, precision=FP64, reps=5):
         """Pick the fastest operator launch shape (HIP library only)."""
@@ -459,7 +463,7 @@ class Context:
         fn.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, _IP, _IP, _IP, _IP, _DP]
         self._check(fn(self.h, ncols, precision, reps, C.byref(cp), C.byref(vt), C.byref(tp),
                        C.byref(gr), _dp(ms)))
-        return dict(cells_per_wg=cp.value, vectors_per_tile=vt.value, tile_pass=tp.value,
+        return dict(cells_per_wg=cp.value, vectors_per_tile=vt.value, kernel=tp.value,
                     gather_rows=gr.value, ms=float(ms[0]))
 
     def profile_apply(self, ncols, precision=FP64, reps=10):

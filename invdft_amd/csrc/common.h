@@ -139,27 +139,51 @@ struct ApplyEpilogue {
 constexpr int kGatherBlocks = 16384;
 // rows of the gather's dot partials for vectors of vbytes-byte elements
 int gather_blocks(const Geo& g, int B, int rows, int vbytes);
+// Cell-product kernels.
+enum CellKernel {
+  CK_ROW_PASS = 0,   // stored H_e, one wave per 16-row block (mirror tiles read again)
+  CK_TILE_PASS = 1,  // stored H_e, every stored tile read once per row half
+  CK_SUM_FACTOR = 2  // no stored H_e: 1D contractions with the per-cell quadrature weights
+};
+// Launch shape of the cell product (chosen by the autotuner).
+struct GemmShape {
+  int cells_per_wg = 1;       // stored-H kernels: cells processed in sequence by one work-group;
+                              // sum-factorised kernel: columns per item (1, 2, 4)
+  int vectors_per_tile = 80;  // stored-H kernels: columns per tile (rounded to 16; >= B: one
+                              // pass); sum-factorised kernel: columns per work-group (2, 4, 8, 16)
+  int gather_rows = 0;        // node rows per gather work-group (0: about kGatherBlocks groups)
+  int kernel = CK_ROW_PASS;   // CellKernel; the tile pass applies to 16-column tiles (FP32 also 32)
+};
+// The cell operator at one precision: the stored upper tiles of H_e (row and
+// tile pass; null until a stored-H kernel first runs) and, for the
+// sum-factorised kernel, the quadrature weights w_e[cell][q] = v_eff JxW with
+// the kinetic scale h/4.  On the uniform mesh
+//   H_e = (h/4) (D (x) M (x) M + M (x) D (x) M + M (x) M (x) D) + L^T diag(w_e) L
+// with the 1D tables of sf_tables.h.
+template <typename T>
+struct CellOp {
+  const T* H = nullptr;
+  const T* W = nullptr;
+  double kin_scale = 0.0;
+};
 // Cell operator in two launches: per-cell H_e (S xcol Xin)_e into the scratch
 // Yc (cell_scratch_elems(g, B) elements), then the node gather + epilogue.
 // Returns the number of gather blocks (rows of dot_partial).
-// Tile shape of the MFMA cell-product kernel (chosen by the autotuner).
-struct GemmShape {
-  int cells_per_wg = 1;       // cells processed in sequence by one work-group
-  int vectors_per_tile = 80;  // columns per launch tile (rounded to 16; >= B: one pass)
-  int gather_rows = 0;        // node rows per gather work-group (0: about kGatherBlocks groups)
-  int tile_pass = 0;          // 1: one pass over the stored tiles (16-column tiles; FP32 also 32)
-};
-// T: cell matrices, cell scratch Yc and MFMA arithmetic; TV: the node-major
+// T: cell operator, cell scratch Yc and arithmetic; TV: the node-major
 // vectors (Xin and the epilogue's blocks).  Sg (type T) scales the input of
 // the cell products, S (type TV) the gathered sums.
 template <typename T, typename TV>
-int launch_cell_apply(const Geo& g, const T* H, const T* Sg, const TV* S, const TV* Xin, T* Yc,
-                      int B, const ApplyEpilogue<TV>& ep, const GemmShape& shape, hipStream_t s);
+int launch_cell_apply(const Geo& g, const CellOp<T>& op, const T* Sg, const TV* S, const TV* Xin,
+                      T* Yc, int B, const ApplyEpilogue<TV>& ep, const GemmShape& shape,
+                      hipStream_t s);
 int64_t cell_scratch_elems(const Geo& g, int B);
 // Phase 1 alone (per-cell products into Yc), for profiling.
 template <typename T>
-void launch_cell_gemm(const Geo& g, const T* H, const T* S, const T* X, const double* xcol, T* Yc,
-                      int B, const GemmShape& shape, hipStream_t s);
+void launch_cell_gemm(const Geo& g, const CellOp<T>& op, const T* S, const T* X,
+                      const double* xcol, T* Yc, int B, const GemmShape& shape, hipStream_t s);
+// Columns per work-group / per item the sum-factorised kernel is built for.
+bool sf_cols_supported(int cols, bool fp64);
+bool sf_vc_supported(int cols, int vc, bool fp64);
 // Y = beta Y + ns[n] (a ca[j] X + b cb[j] Z); null pointers read as 1 (X/Z as 0).
 template <typename T>
 void launch_lincomb(int64_t N, int B, T* Y, double beta, const T* X, double a, const double* ca,

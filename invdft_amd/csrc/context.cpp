@@ -13,6 +13,8 @@
 #include <type_traits>
 
 #include "common.h"
+#include "sf_tables.h"
+#include "sf_tables.h"
 
 using namespace idft;
 
@@ -71,10 +73,11 @@ struct idft_ctx {
   DevBuf<double> potBase, rhoT, weight, veff, rho, u;
   std::vector<double> hMdiag;
   // Hamiltonian
-  DevBuf<double> H64, Yc64;
-  DevBuf<float> H32, Yc32;
-  GemmShape shape64, shape32;  // cell-product tile shapes (autotuned)
-  bool haveH = false;
+  DevBuf<double> H64, Yc64, W64;
+  DevBuf<float> H32, Yc32, W32;
+  GemmShape shape64, shape32;  // cell-product launch shapes (autotuned)
+  bool haveH = false;          // operator of the current potential set (W64/W32)
+  bool storedH = false;        // H64/H32 built for it (only the stored-H kernels read them)
   // eigen state (Loewdin vectors) and work
   DevBuf<double> X, X2, HX, Xtmp, T2buf, rref, xdelta, psiAll, force;
   DevBuf<float> F0, F1, F2, Rf;
@@ -114,24 +117,43 @@ struct idft_ctx {
       Yc64.resize(cell_scratch_elems(g, b));
       st.applies_fp64++;
       st.apply_cols_fp64 += b;
-      return launch_cell_apply<double, double>(g, H64.p, S.p, S.p, Xin, Yc64.p, b, ep, shape64, s);
+      return launch_cell_apply<double, double>(g, op64(), S.p, S.p, Xin, Yc64.p, b, ep, shape64,
+                                               s);
     } else {
-      if (!H32.p) throw Error(IDFT_ESTATE, "FP32 Hamiltonian not built (precision is FP64)");
       Yc32.resize(cell_scratch_elems(g, b));
       st.applies_fp32++;
       st.apply_cols_fp32 += b;
-      return launch_cell_apply<float, float>(g, H32.p, S32.p, S32.p, Xin, Yc32.p, b, ep, shape32,
+      return launch_cell_apply<float, float>(g, op32(), S32.p, S32.p, Xin, Yc32.p, b, ep, shape32,
                                              s);
     }
   }
-  // FP32 cell operator (FP32 H_e, MFMA and cell scratch) on FP64 vectors:
-  // input scaled and rounded once, gathered sums and epilogue in FP64.
+  // FP32 cell operator (FP32 H_e or weights, arithmetic and cell scratch) on
+  // FP64 vectors: input scaled and rounded once, gathered sums and epilogue
+  // in FP64.
   int apply_f32op(const double* Xin, int b, const ApplyEpilogue<double>& ep) {
-    if (!H32.p) throw Error(IDFT_ESTATE, "FP32 Hamiltonian not built (precision is FP64)");
     Yc32.resize(cell_scratch_elems(g, b));
     st.applies_fp32++;
     st.apply_cols_fp32 += b;
-    return launch_cell_apply<float, double>(g, H32.p, S32.p, S.p, Xin, Yc32.p, b, ep, shape32, s);
+    return launch_cell_apply<float, double>(g, op32(), S32.p, S.p, Xin, Yc32.p, b, ep, shape32, s);
+  }
+  // The cell operator at each precision for the current shapes; the stored
+  // cell matrices are built on first use by a stored-H kernel.
+  CellOp<double> op64() {
+    if (shape64.kernel != CK_SUM_FACTOR) ensure_stored_h();
+    CellOp<double> o;
+    o.H = H64.p;
+    o.W = W64.p;
+    o.kin_scale = 0.25 * t.h;
+    return o;
+  }
+  CellOp<float> op32() {
+    if (!mixed) throw Error(IDFT_ESTATE, "FP32 Hamiltonian not built (precision is FP64)");
+    if (shape32.kernel != CK_SUM_FACTOR) ensure_stored_h();
+    CellOp<float> o;
+    o.H = H32.p;
+    o.W = W32.p;
+    o.kin_scale = 0.25 * t.h;
+    return o;
   }
   // Y = alpha S H S X + beta Y
   template <typename T>
@@ -178,14 +200,56 @@ struct idft_ctx {
   void zero_boundary(double* v, int b) { launch_zero_rows(t.N, b, v, bnd.p, s); }
 
   // ------------------------------------------------------ Hamiltonian
+  // Operator of the current v_eff: the quadrature weights w = v_eff JxW
+  // (FP64, and FP32 in MIXED); the stored H_e waits for a kernel that reads it.
   void build_h() {
+    W64.resize(nQ);
+    launch_mul_q(nQ, t.nq, veff.p, jxw.p, W64.p, s);
+    if (mixed) {
+      W32.resize(nQ);
+      launch_convert_d2f(nQ, W32.p, W64.p, s);
+    }
+    haveH = true;
+    storedH = false;
+    haveHX = false;
+  }
+  void ensure_stored_h() {
+    if (storedH) return;
+    if (!haveH) throw Error(IDFT_ESTATE, "no Hamiltonian: call set_potential first");
     const double hh = 0.5 * t.h;
     H64.resize((size_t)t.ncell * hamiltonian_tile_elems(t.p));
     if (mixed) H32.resize((size_t)t.ncell * hamiltonian_tile_elems(t.p));
     launch_build_hamiltonian(g, dt, hh, veff.p, jxw.p, H64.p, mixed ? H32.p : nullptr, s);
-    haveH = true;
-    haveHX = false;
+    storedH = true;
   }
+  // Frees the stored cell matrices when neither precision's kernel reads them.
+  void release_unused_h() {
+    if (shape64.kernel != CK_SUM_FACTOR || (mixed && shape32.kernel != CK_SUM_FACTOR)) return;
+    H64.release();
+    H32.release();
+    storedH = false;
+  }
+  // The sum-factorised kernel's compile-time 1D tables against this
+  // context's FE tables (fe.cpp): they must agree to rounding.
+  void check_sf_tables() const {
+    double d = 0.0;
+    auto cmp = [&](const double* L, const double* M, const double* D) {
+      for (int i = 0; i < t.q1 * t.n1; ++i) d = std::max(d, std::fabs(L[i] - t.Lq[i]));
+      for (int i = 0; i < t.n1 * t.n1; ++i) {
+        d = std::max(d, std::fabs(M[i] - t.M1[i]));
+        d = std::max(d, std::fabs(D[i] - t.D1[i]) / 16.0);
+      }
+    };
+    switch (t.p) {
+      case 2: cmp(SfC<2>::v.L, SfC<2>::v.M, SfC<2>::v.D); break;
+      case 3: cmp(SfC<3>::v.L, SfC<3>::v.M, SfC<3>::v.D); break;
+      case 4: cmp(SfC<4>::v.L, SfC<4>::v.M, SfC<4>::v.D); break;
+      case 5: cmp(SfC<5>::v.L, SfC<5>::v.M, SfC<5>::v.D); break;
+      default: break;
+    }
+    if (!(d < 1e-13)) throw Error(IDFT_EINVAL, "sum-factorisation tables disagree with the FE tables");
+  }
+
   void set_potential(const double* pot_host) {
     DevBuf<double> pot;
     pot.upload(pot_host, (size_t)t.NV, s);
@@ -648,7 +712,11 @@ struct idft_ctx {
     launch_lincomb<double>(N, b, W.r2.p, 0.0, rhs, 1.0, nullptr, nullptr, 0.0, nullptr, nullptr, s);
     launch_lincomb<double>(N, b, W.w1.p, 0.0, nullptr, 0.0, nullptr, nullptr, 0.0, nullptr, nullptr, s);
     launch_lincomb<double>(N, b, W.w.p, 0.0, nullptr, 0.0, nullptr, nullptr, 0.0, nullptr, nullptr, s);
-    const bool canRelax = relax > 0.0 && H32.p != nullptr;
+    const bool canRelax = relax > 0.0 && mixed;
+    // stored cell matrices (if a stored-H kernel is selected) built before
+    // any step is captured into a graph
+    (void)op64();
+    if (canRelax) (void)op32();
     int nblk = std::max(1024, gather_blocks(g, b, shape64.gather_rows, 8));
     if (canRelax) nblk = std::max(nblk, gather_blocks(g, b, shape32.gather_rows, 8));
     partial.resize((size_t)nblk * b);
@@ -983,6 +1051,17 @@ idft_ctx* create_ctx(const idft_desc* d, const double* pot_base_q, const double*
     c->dt.D1.upload(t.D1.data(), t.D1.size(), s);
     c->dt.Wq.upload(t.Wq.data(), t.Wq.size(), s);
     c->dt.Wc.upload(t.Wc.data(), t.Wc.size(), s);
+    c->check_sf_tables();
+    // default cell product: sum-factorised (no stored H_e) for blocks of up
+    // to 32 columns, where it is the faster kernel; the row pass over stored
+    // tiles for wider blocks (idft_autotune_apply refines both)
+    if (d->n_eig <= 32) {
+      for (GemmShape* sh : {&c->shape64, &c->shape32}) {
+        sh->kernel = CK_SUM_FACTOR;
+        sh->vectors_per_tile = 4;
+        sh->cells_per_wg = 1;
+      }
+    }
     // uniform-cell JxW at the parent Gauss points
     const double hh = 0.5 * t.h, J = hh * hh * hh;
     std::vector<double> jw(t.nq);
@@ -1336,8 +1415,8 @@ int idft_profile_apply(idft_ctx* c, int32_t ncols, int32_t prec, int32_t reps, d
       else c->happly64(xd.p, yd.p, ncols, 1.0);
     };
     auto gemm = [&] {
-      if (f32) launch_cell_gemm<float>(c->g, c->H32.p, c->S32.p, xf.p, nullptr, c->Yc32.p, ncols, c->shape32, c->s);
-      else launch_cell_gemm<double>(c->g, c->H64.p, c->S.p, xd.p, nullptr, c->Yc64.p, ncols, c->shape64, c->s);
+      if (f32) launch_cell_gemm<float>(c->g, c->op32(), c->S32.p, xf.p, nullptr, c->Yc32.p, ncols, c->shape32, c->s);
+      else launch_cell_gemm<double>(c->g, c->op64(), c->S.p, xd.p, nullptr, c->Yc64.p, ncols, c->shape64, c->s);
     };
     hipEvent_t e0, e1;
     IDFT_HIP(hipEventCreate(&e0));
@@ -1377,26 +1456,29 @@ int idft_bfgs_set_checkpoint(idft_ctx* c, const char* path, int32_t every, int32
 }
 
 int idft_set_apply_shape(idft_ctx* c, int32_t prec, int32_t cells_per_wg,
-                         int32_t vectors_per_tile, int32_t tile_pass, int32_t gather_rows) {
-  if (!c || cells_per_wg < 1 || cells_per_wg > 64 || vectors_per_tile < 16 ||
-      vectors_per_tile > kMaxCols || vectors_per_tile % 16 || gather_rows < 0 ||
-      gather_rows > 4096 || tile_pass < 0 || tile_pass > 1)
+                         int32_t vectors_per_tile, int32_t kernel, int32_t gather_rows) {
+  if (!c || cells_per_wg < 1 || cells_per_wg > 64 || gather_rows < 0 || gather_rows > 4096 ||
+      kernel < CK_ROW_PASS || kernel > CK_SUM_FACTOR)
+    return IDFT_EINVAL;
+  if (kernel == CK_SUM_FACTOR
+          ? !sf_vc_supported(vectors_per_tile, cells_per_wg, prec != IDFT_MIXED)
+          : (vectors_per_tile < 16 || vectors_per_tile > kMaxCols || vectors_per_tile % 16))
     return IDFT_EINVAL;
   GemmShape& t = prec == IDFT_MIXED ? c->shape32 : c->shape64;
   t.cells_per_wg = cells_per_wg;
   t.vectors_per_tile = vectors_per_tile;
-  t.tile_pass = tile_pass;
+  t.kernel = kernel;
   t.gather_rows = gather_rows;
   return IDFT_OK;
 }
 
 int idft_autotune_apply(idft_ctx* c, int32_t ncols, int32_t prec, int32_t reps,
-                        int32_t* cells_per_wg, int32_t* vectors_per_tile, int32_t* tile_pass,
+                        int32_t* cells_per_wg, int32_t* vectors_per_tile, int32_t* kernel,
                         int32_t* gather_rows, double* ms_best) {
   if (!c || ncols < 1 || ncols > kMaxCols || reps < 1) return IDFT_EINVAL;
   if (!c->haveH) { c->err = "no Hamiltonian"; return IDFT_ESTATE; }
   const bool f32 = prec == IDFT_MIXED;
-  if (f32 && !c->H32.p) { c->err = "FP32 Hamiltonian not built"; return IDFT_ESTATE; }
+  if (f32 && !c->mixed) { c->err = "FP32 Hamiltonian not built"; return IDFT_ESTATE; }
   return guarded(c, [&] {
     GemmShape& target = f32 ? c->shape32 : c->shape64;
     const GemmShape saved = target;
@@ -1415,13 +1497,23 @@ int idft_autotune_apply(idft_ctx* c, int32_t ncols, int32_t prec, int32_t reps,
       if (ms < bestMs) { bestMs = ms; best = target; }
     };
     target.gather_rows = 0;
+    target.kernel = CK_SUM_FACTOR;
+    for (int cg : {2, 4, 8, 16}) {
+      if (cg > 2 && cg / 2 >= ncols) continue;  // mostly empty column groups
+      for (int vc : {1, 2, 4}) {
+        if (!sf_vc_supported(cg, vc, !f32)) continue;
+        target.vectors_per_tile = cg;
+        target.cells_per_wg = vc;
+        trial();
+      }
+    }
     const int tpMax = f32 ? 32 : 16;  // widest tile of the tile-pass kernel
     for (int vt : widths) {
-      for (int tp = 0; tp <= (vt <= tpMax ? 1 : 0); ++tp) {
+      for (int k = CK_ROW_PASS; k <= (vt <= tpMax ? CK_TILE_PASS : CK_ROW_PASS); ++k) {
         for (int cp : {1, 2, 4, 8}) {
           target.cells_per_wg = cp;
           target.vectors_per_tile = vt;
-          target.tile_pass = tp;
+          target.kernel = k;
           trial();
         }
       }
@@ -1432,9 +1524,10 @@ int idft_autotune_apply(idft_ctx* c, int32_t ncols, int32_t prec, int32_t reps,
       trial();
     }
     target = best;
+    c->release_unused_h();
     if (cells_per_wg) *cells_per_wg = best.cells_per_wg;
     if (vectors_per_tile) *vectors_per_tile = best.vectors_per_tile;
-    if (tile_pass) *tile_pass = best.tile_pass;
+    if (kernel) *kernel = best.kernel;
     if (gather_rows) *gather_rows = best.gather_rows;
     if (ms_best) *ms_best = bestMs;
   });

@@ -15,6 +15,7 @@
 #include <type_traits>
 
 #include "common.h"
+#include "sf_tables.h"
 
 namespace idft {
 
@@ -460,6 +461,269 @@ __global__ __launch_bounds__(kSymmThreads) void k_cell_symm(Geo g, const T* __re
     }
   }
 }
+
+// ----------------------------------------------- sum-factorised cell product
+// Phase 1 without stored cell matrices.  On the uniform mesh
+//   H_e = (h/4)(D(x)M(x)M + M(x)D(x)M + M(x)M(x)D) + L^T diag(w_e) L,  L = Lq(x)Lq(x)Lq,
+// so H_e X_e is a chain of 1D contractions along x, y and z: 23 k FMA per
+// column at p = 5 (the dense product is 47 k), and per cell only the (p+3)^3
+// weights w_e are read from HBM instead of n_e(n_e+1)/2 entries of H_e.  The
+// 1D tables are compile-time constants (SfC<P>), so after unrolling every
+// coefficient is an immediate operand; h/4 scales the kinetic sum once.
+//
+// One work-group handles one cell and CG columns; an item is one line of the
+// cell's tensor along one axis for VC consecutive columns (VC-wide LDS
+// accesses).  Intermediates live in LDS as [entry][CG / VC] packs; the
+// x-line arrays have padded line strides (Q+1, N1+1 entries) so that the 64
+// lanes of a wave spread over the banks:
+//   1  x lines (c,b):   t1[c][b][qx] = Lq x,  ka = M x,  kb = D x   (x gathered from HBM)
+//   2  y lines:         t2[c][qy][qx] = Lq t1;  k1 = M ka,  k2 = D ka + M kb
+//   3  z lines:         u = w .* (Lq t2),  s2[c][qy][qx] = Lq^T u;  yk = D k1 + M k2
+//   4  y lines back:    s1[c][b][qx] = Lq^T s2
+//   5  x lines back:    Yc[e][(c,b,a)][j] = (Lq^T s1)[a] + (h/4) yk[a]
+// The regions r1 / r2 are reused: (t1, ka, kb) -> (s2, yk) in r1 and
+// (t2, k1, k2) -> s1 in r2, each rewritten only after the barrier that ends
+// the stage reading it.
+template <typename T, typename TV>
+__device__ inline T cell_input(const T* __restrict__ S, const TV* __restrict__ X,
+                               const double* __restrict__ xn, const double* __restrict__ xcol,
+                               int64_t n, int B, int col) {
+  if constexpr (std::is_same<T, TV>::value) {
+    T v = S[n] * X[n * B + col];
+    if (xn) v *= (T)xn[n];
+    if (xcol) v *= (T)xcol[col];
+    return v;
+  } else {  // wider input vectors: scale in FP64, round once
+    double w = (double)S[n] * (double)X[n * B + col];
+    if (xn) w *= xn[n];
+    if (xcol) w *= xcol[col];
+    return (T)w;
+  }
+}
+
+__host__ __device__ constexpr int cmax(int a, int b) { return a > b ? a : b; }
+
+template <typename T, int VC>
+struct alignas(sizeof(T) * VC) Pack {
+  T v[VC];
+};
+template <typename T, int VC>
+__device__ inline void pack_zero(Pack<T, VC>& p) {
+#pragma unroll
+  for (int u = 0; u < VC; ++u) p.v[u] = T(0);
+}
+template <typename T, int VC>
+__device__ inline void pack_fma(Pack<T, VC>& acc, T c, const Pack<T, VC>& x) {
+#pragma unroll
+  for (int u = 0; u < VC; ++u) acc.v[u] += c * x.v[u];
+}
+
+// Compile-time coefficients of the 1D tables (constant expressions, so each
+// is an immediate operand rather than a load from a constant global).
+template <typename T, int P, int I>
+__device__ constexpr T sfL() { return (T)SfC<P>::v.L[I]; }
+template <typename T, int P, int I>
+__device__ constexpr T sfM() { return (T)SfC<P>::v.M[I]; }
+template <typename T, int P, int I>
+__device__ constexpr T sfD() { return (T)SfC<P>::v.D[I]; }
+#define IDFT_IC(x) decltype(x)::value
+
+template <typename T, typename TV, int P, int CG, int VC>
+__global__ __launch_bounds__(kThreads) void k_cell_sf(Geo g, T hq, const T* __restrict__ W,
+                                                      const T* __restrict__ S,
+                                                      const TV* __restrict__ X,
+                                                      const double* __restrict__ xn,
+                                                      const double* __restrict__ xcol,
+                                                      T* __restrict__ Yc, int B) {
+  constexpr int N1 = P + 1, Q = P + 3, N2 = N1 * N1, NE = N2 * N1, Q2 = Q * Q, Q3 = Q2 * Q;
+  constexpr int QP = Q + 1, N1P = N1 + 1, NV = CG / VC;
+  constexpr int SZ1 = cmax(N2 * QP + 2 * N2 * N1P, N1 * Q2 + N2 * N1P);
+  constexpr int SZ2 = cmax(N1 * Q2 + 2 * NE, N2 * QP);
+  typedef Pack<T, VC> pk;
+  __shared__ pk r1[SZ1 * NV];
+  __shared__ pk r2[SZ2 * NV];
+  // cells dealt to the 8 XCDs in contiguous runs
+  const int64_t per = (g.ncell + 7) / 8;
+  const int64_t e = (blockIdx.x % 8) * per + blockIdx.x / 8;
+  if (e >= g.ncell) return;
+  const int c0 = blockIdx.y * CG;
+  const int bw = min(CG, B - c0);
+  const int tid = threadIdx.x;
+  const int64_t cx = e % g.nc, cy = (e / g.nc) % g.nc, cz = e / ((int64_t)g.nc * g.nc);
+  pk* t1 = r1;                   // [N2][QP]
+  pk* ka = r1 + N2 * QP * NV;    // [N2][N1P]
+  pk* kb = ka + N2 * N1P * NV;   // [N2][N1P]
+  pk* t2 = r2;                   // [N1][Q][Q]
+  pk* k1 = r2 + N1 * Q2 * NV;    // [N1][N1][N1]
+  pk* k2 = k1 + NE * NV;         // [N1][N1][N1]
+  pk* s2 = r1;                   // [N1][Q][Q]
+  pk* yk = r1 + N1 * Q2 * NV;    // [N2][N1P]
+  pk* s1 = r2;                   // [N2][QP]
+  const T* we = W + e * Q3;
+  // stage 1: x lines ln = c*N1 + b
+  for (int it = tid; it < N2 * NV; it += kThreads) {
+    const int jv = it % NV, ln = it / NV;
+    pk v[N1];
+    const int b = ln % N1, c = ln / N1;
+    const int64_t n0 = ((cz * g.p + c) * g.N1 + (cy * g.p + b)) * (int64_t)g.N1 + cx * g.p;
+#pragma unroll
+    for (int a = 0; a < N1; ++a)
+#pragma unroll
+      for (int u = 0; u < VC; ++u) {
+        const int j = jv * VC + u;
+        v[a].v[u] = j < bw ? cell_input<T, TV>(S, X, xn, xcol, n0 + a, B, c0 + j) : T(0);
+      }
+    static_for<0, Q>([&](auto q) {
+      pk s;
+      pack_zero(s);
+      static_for<0, N1>([&](auto a) {
+        pack_fma(s, sfL<T, P, IDFT_IC(q) * N1 + IDFT_IC(a)>(), v[IDFT_IC(a)]);
+      });
+      t1[(ln * QP + IDFT_IC(q)) * NV + jv] = s;
+    });
+    static_for<0, N1>([&](auto a2) {
+      pk sm, sd;
+      pack_zero(sm);
+      pack_zero(sd);
+      static_for<0, N1>([&](auto a) {
+        pack_fma(sm, sfM<T, P, IDFT_IC(a2) * N1 + IDFT_IC(a)>(), v[IDFT_IC(a)]);
+        pack_fma(sd, sfD<T, P, IDFT_IC(a2) * N1 + IDFT_IC(a)>(), v[IDFT_IC(a)]);
+      });
+      ka[(ln * N1P + IDFT_IC(a2)) * NV + jv] = sm;
+      kb[(ln * N1P + IDFT_IC(a2)) * NV + jv] = sd;
+    });
+  }
+  __syncthreads();
+  // stage 2: y lines -- potential (c, qx), then kinetic (c, a)
+  for (int it = tid; it < (N1 * Q + N2) * NV; it += kThreads) {
+    const int jv = it % NV, ln = it / NV;
+    if (ln < N1 * Q) {
+      const int qx = ln % Q, c = ln / Q;
+      pk v[N1];
+#pragma unroll
+      for (int b = 0; b < N1; ++b) v[b] = t1[((c * N1 + b) * QP + qx) * NV + jv];
+      static_for<0, Q>([&](auto qy) {
+        pk s;
+        pack_zero(s);
+        static_for<0, N1>([&](auto b) {
+          pack_fma(s, sfL<T, P, IDFT_IC(qy) * N1 + IDFT_IC(b)>(), v[IDFT_IC(b)]);
+        });
+        t2[((c * Q + IDFT_IC(qy)) * Q + qx) * NV + jv] = s;
+      });
+    } else {
+      const int l2 = ln - N1 * Q;
+      const int a = l2 % N1, c = l2 / N1;
+      pk va[N1], vb[N1];
+#pragma unroll
+      for (int b = 0; b < N1; ++b) {
+        va[b] = ka[((c * N1 + b) * N1P + a) * NV + jv];
+        vb[b] = kb[((c * N1 + b) * N1P + a) * NV + jv];
+      }
+      static_for<0, N1>([&](auto b2) {
+        pk s1v, s2v;
+        pack_zero(s1v);
+        pack_zero(s2v);
+        static_for<0, N1>([&](auto b) {
+          constexpr int I = IDFT_IC(b2) * N1 + IDFT_IC(b);
+          pack_fma(s1v, sfM<T, P, I>(), va[IDFT_IC(b)]);
+          pack_fma(s2v, sfD<T, P, I>(), va[IDFT_IC(b)]);
+          pack_fma(s2v, sfM<T, P, I>(), vb[IDFT_IC(b)]);
+        });
+        k1[((c * N1 + IDFT_IC(b2)) * N1 + a) * NV + jv] = s1v;
+        k2[((c * N1 + IDFT_IC(b2)) * N1 + a) * NV + jv] = s2v;
+      });
+    }
+  }
+  __syncthreads();
+  // stage 3: z lines -- potential (qy, qx) through the quadrature weights,
+  // then kinetic (b, a)
+  for (int it = tid; it < (Q2 + N2) * NV; it += kThreads) {
+    const int jv = it % NV, ln = it / NV;
+    if (ln < Q2) {
+      pk v[N1];
+#pragma unroll
+      for (int c = 0; c < N1; ++c) v[c] = t2[(c * Q2 + ln) * NV + jv];
+      pk u[Q];
+      static_for<0, Q>([&](auto qz) {
+        pk s;
+        pack_zero(s);
+        static_for<0, N1>([&](auto c) {
+          pack_fma(s, sfL<T, P, IDFT_IC(qz) * N1 + IDFT_IC(c)>(), v[IDFT_IC(c)]);
+        });
+        const T wq = we[IDFT_IC(qz) * Q2 + ln];
+#pragma unroll
+        for (int uu = 0; uu < VC; ++uu) u[IDFT_IC(qz)].v[uu] = s.v[uu] * wq;
+      });
+      static_for<0, N1>([&](auto c) {
+        pk s;
+        pack_zero(s);
+        static_for<0, Q>([&](auto qz) {
+          pack_fma(s, sfL<T, P, IDFT_IC(qz) * N1 + IDFT_IC(c)>(), u[IDFT_IC(qz)]);
+        });
+        s2[(IDFT_IC(c) * Q2 + ln) * NV + jv] = s;
+      });
+    } else {
+      const int l2 = ln - Q2;  // b * N1 + a
+      const int a = l2 % N1, b = l2 / N1;
+      pk v1[N1], v2[N1];
+#pragma unroll
+      for (int c = 0; c < N1; ++c) {
+        v1[c] = k1[(c * N2 + l2) * NV + jv];
+        v2[c] = k2[(c * N2 + l2) * NV + jv];
+      }
+      static_for<0, N1>([&](auto c2) {
+        pk s;
+        pack_zero(s);
+        static_for<0, N1>([&](auto c) {
+          constexpr int I = IDFT_IC(c2) * N1 + IDFT_IC(c);
+          pack_fma(s, sfD<T, P, I>(), v1[IDFT_IC(c)]);
+          pack_fma(s, sfM<T, P, I>(), v2[IDFT_IC(c)]);
+        });
+        yk[((IDFT_IC(c2) * N1 + b) * N1P + a) * NV + jv] = s;
+      });
+    }
+  }
+  __syncthreads();
+  // stage 4: y lines back (c, qx)
+  for (int it = tid; it < N1 * Q * NV; it += kThreads) {
+    const int jv = it % NV, ln = it / NV;
+    const int qx = ln % Q, c = ln / Q;
+    pk v[Q];
+#pragma unroll
+    for (int qy = 0; qy < Q; ++qy) v[qy] = s2[((c * Q + qy) * Q + qx) * NV + jv];
+    static_for<0, N1>([&](auto b) {
+      pk s;
+      pack_zero(s);
+      static_for<0, Q>([&](auto qy) {
+        pack_fma(s, sfL<T, P, IDFT_IC(qy) * N1 + IDFT_IC(b)>(), v[IDFT_IC(qy)]);
+      });
+      s1[((c * N1 + IDFT_IC(b)) * QP + qx) * NV + jv] = s;
+    });
+  }
+  __syncthreads();
+  // stage 5: x lines back (c, b) plus the kinetic part, into the cell scratch
+  for (int it = tid; it < N2 * NV; it += kThreads) {
+    const int jv = it % NV, ln = it / NV;
+    if (jv * VC >= bw) continue;
+    pk v[Q];
+#pragma unroll
+    for (int qx = 0; qx < Q; ++qx) v[qx] = s1[(ln * QP + qx) * NV + jv];
+    T* yc = Yc + ((size_t)e * NE + ln * N1) * B + c0 + jv * VC;
+    static_for<0, N1>([&](auto a) {
+      const pk kin = yk[(ln * N1P + IDFT_IC(a)) * NV + jv];
+      pk s;
+#pragma unroll
+      for (int u = 0; u < VC; ++u) s.v[u] = hq * kin.v[u];
+      static_for<0, Q>([&](auto qx) {
+        pack_fma(s, sfL<T, P, IDFT_IC(qx) * N1 + IDFT_IC(a)>(), v[IDFT_IC(qx)]);
+      });
+#pragma unroll
+      for (int u = 0; u < VC; ++u)
+        if (jv * VC + u < bw) yc[(size_t)IDFT_IC(a) * B + u] = s.v[u];
+    });
+  }
+}
+#undef IDFT_IC
 
 // Phase 2: every node sums its contributions from the (at most 8) cells that
 // share it -- found arithmetically on the structured mesh, so no atomics and
@@ -1391,16 +1655,79 @@ void launch_build_hamiltonian(const Geo& g, const DevTables& t, double hh, const
   IDFT_HIP(hipGetLastError());
 }
 
+// Columns per work-group (CG) and per item (VC) of the sum-factorised
+// kernel: its LDS holds about 1644 CG values at p = 5, so FP64 stops at 8
+// columns; VC in {1, 2, 4} divides CG, FP64 up to 2.
+bool sf_cols_supported(int cols, bool fp64) {
+  return cols == 2 || cols == 4 || cols == 8 || (!fp64 && cols == 16);
+}
+bool sf_vc_supported(int cols, int vc, bool fp64) {
+  return sf_cols_supported(cols, fp64) && (vc == 1 || vc == 2 || (!fp64 && vc == 4)) &&
+         vc <= cols;
+}
+
+template <typename T, typename TV, int P, int CG>
+static void cell_sf_cg(const Geo& g, const CellOp<T>& op, const T* S, const TV* X,
+                       const double* xn, const double* xcol, T* Yc, int B, int vc,
+                       hipStream_t s) {
+  dim3 grid((unsigned)(8 * ((g.ncell + 7) / 8)), (unsigned)((B + CG - 1) / CG)), block(kThreads);
+  const T hq = (T)op.kin_scale;
+  if (vc == 1) {
+    k_cell_sf<T, TV, P, CG, 1><<<grid, block, 0, s>>>(g, hq, op.W, S, X, xn, xcol, Yc, B);
+    return;
+  }
+  if (vc == 2) {
+    k_cell_sf<T, TV, P, CG, 2><<<grid, block, 0, s>>>(g, hq, op.W, S, X, xn, xcol, Yc, B);
+    return;
+  }
+  if constexpr (std::is_same<T, float>::value && CG >= 4) {
+    if (vc == 4) {
+      k_cell_sf<T, TV, P, CG, 4><<<grid, block, 0, s>>>(g, hq, op.W, S, X, xn, xcol, Yc, B);
+      return;
+    }
+  }
+  throw Error(IDFT_EINVAL, "columns per item not supported by the sum-factorised kernel");
+}
+
 template <typename T, typename TV, int P>
-static void cell_gemm_p(const Geo& g, const T* H, const T* S, const TV* X, const double* xn,
-                        const double* xcol, T* Yc, int B, const GemmShape& shape, hipStream_t s) {
+static void cell_sf_p(const Geo& g, const CellOp<T>& op, const T* S, const TV* X,
+                      const double* xn, const double* xcol, T* Yc, int B, int cg, int vc,
+                      hipStream_t s) {
+  if (!op.W) throw Error(IDFT_ESTATE, "quadrature weights of the cell operator not set");
+  if (!sf_vc_supported(cg, vc, std::is_same<T, double>::value))
+    throw Error(IDFT_EINVAL, "shape not supported by the sum-factorised kernel");
+  switch (cg) {
+    case 2: cell_sf_cg<T, TV, P, 2>(g, op, S, X, xn, xcol, Yc, B, vc, s); break;
+    case 4: cell_sf_cg<T, TV, P, 4>(g, op, S, X, xn, xcol, Yc, B, vc, s); break;
+    case 8: cell_sf_cg<T, TV, P, 8>(g, op, S, X, xn, xcol, Yc, B, vc, s); break;
+    case 16:
+      if constexpr (std::is_same<T, float>::value) {
+        cell_sf_cg<T, TV, P, 16>(g, op, S, X, xn, xcol, Yc, B, vc, s);
+        break;
+      }
+      [[fallthrough]];
+    default: throw Error(IDFT_EINVAL, "columns per work-group not supported by the sum-factorised kernel");
+  }
+}
+
+template <typename T, typename TV, int P>
+static void cell_gemm_p(const Geo& g, const CellOp<T>& op, const T* S, const TV* X,
+                        const double* xn, const double* xcol, T* Yc, int B,
+                        const GemmShape& shape, hipStream_t s) {
+  if (shape.kernel == CK_SUM_FACTOR) {
+    cell_sf_p<T, TV, P>(g, op, S, X, xn, xcol, Yc, B, shape.vectors_per_tile,
+                        shape.cells_per_wg, s);
+    return;
+  }
+  const T* H = op.H;
+  if (!H) throw Error(IDFT_ESTATE, "stored cell matrices not built");
   const int vt = std::min(shape.vectors_per_tile, ((B + 15) / 16) * 16);
   const int nct = std::max(1, std::min(5, vt / 16));
   const int cpw = std::max(1, shape.cells_per_wg);
   const int64_t nwg = (g.ncell + cpw - 1) / cpw;
   dim3 grid((unsigned)(8 * ((nwg + 7) / 8)), (unsigned)((B + 16 * nct - 1) / (16 * nct))),
       block(kThreads);
-  if (shape.tile_pass && nct <= (std::is_same<T, float>::value ? 2 : 1)) {
+  if (shape.kernel == CK_TILE_PASS && nct <= (std::is_same<T, float>::value ? 2 : 1)) {
     const dim3 block2(kSymmThreads);
     if (nct == 1) k_cell_symm<T, TV, P, 1><<<grid, block2, 0, s>>>(g, H, S, X, xn, xcol, Yc, B, cpw);
     if constexpr (std::is_same<T, float>::value) {
@@ -1450,32 +1777,35 @@ int gather_blocks(const Geo& g, int B, int rows, int vbytes) {
 }
 
 template <typename T>
-void launch_cell_gemm(const Geo& g, const T* H, const T* S, const T* X, const double* xcol, T* Yc,
-                      int B, const GemmShape& shape, hipStream_t s) {
+void launch_cell_gemm(const Geo& g, const CellOp<T>& op, const T* S, const T* X,
+                      const double* xcol, T* Yc, int B, const GemmShape& shape, hipStream_t s) {
   if (B < 1 || B > kMaxCols) throw Error(IDFT_EINVAL, "block width out of range");
   switch (g.p) {
-    case 2: cell_gemm_p<T, T, 2>(g, H, S, X, nullptr, xcol, Yc, B, shape, s); break;
-    case 3: cell_gemm_p<T, T, 3>(g, H, S, X, nullptr, xcol, Yc, B, shape, s); break;
-    case 4: cell_gemm_p<T, T, 4>(g, H, S, X, nullptr, xcol, Yc, B, shape, s); break;
-    case 5: cell_gemm_p<T, T, 5>(g, H, S, X, nullptr, xcol, Yc, B, shape, s); break;
+    case 2: cell_gemm_p<T, T, 2>(g, op, S, X, nullptr, xcol, Yc, B, shape, s); break;
+    case 3: cell_gemm_p<T, T, 3>(g, op, S, X, nullptr, xcol, Yc, B, shape, s); break;
+    case 4: cell_gemm_p<T, T, 4>(g, op, S, X, nullptr, xcol, Yc, B, shape, s); break;
+    case 5: cell_gemm_p<T, T, 5>(g, op, S, X, nullptr, xcol, Yc, B, shape, s); break;
     default: throw Error(IDFT_EINVAL, "degree not supported by the HIP kernels (2..5)");
   }
   IDFT_HIP(hipGetLastError());
 }
-template void launch_cell_gemm<float>(const Geo&, const float*, const float*, const float*,
-                                      const double*, float*, int, const GemmShape&, hipStream_t);
-template void launch_cell_gemm<double>(const Geo&, const double*, const double*, const double*,
-                                       const double*, double*, int, const GemmShape&, hipStream_t);
+template void launch_cell_gemm<float>(const Geo&, const CellOp<float>&, const float*,
+                                      const float*, const double*, float*, int,
+                                      const GemmShape&, hipStream_t);
+template void launch_cell_gemm<double>(const Geo&, const CellOp<double>&, const double*,
+                                       const double*, const double*, double*, int,
+                                       const GemmShape&, hipStream_t);
 
 template <typename T, typename TV>
-int launch_cell_apply(const Geo& g, const T* H, const T* Sg, const TV* S, const TV* X, T* Yc,
-                      int B, const ApplyEpilogue<TV>& ep, const GemmShape& shape, hipStream_t s) {
+int launch_cell_apply(const Geo& g, const CellOp<T>& op, const T* Sg, const TV* S, const TV* X,
+                      T* Yc, int B, const ApplyEpilogue<TV>& ep, const GemmShape& shape,
+                      hipStream_t s) {
   if (B < 1 || B > kMaxCols) throw Error(IDFT_EINVAL, "block width out of range");
   switch (g.p) {
-    case 2: cell_gemm_p<T, TV, 2>(g, H, Sg, X, ep.xn, ep.xcol, Yc, B, shape, s); break;
-    case 3: cell_gemm_p<T, TV, 3>(g, H, Sg, X, ep.xn, ep.xcol, Yc, B, shape, s); break;
-    case 4: cell_gemm_p<T, TV, 4>(g, H, Sg, X, ep.xn, ep.xcol, Yc, B, shape, s); break;
-    case 5: cell_gemm_p<T, TV, 5>(g, H, Sg, X, ep.xn, ep.xcol, Yc, B, shape, s); break;
+    case 2: cell_gemm_p<T, TV, 2>(g, op, Sg, X, ep.xn, ep.xcol, Yc, B, shape, s); break;
+    case 3: cell_gemm_p<T, TV, 3>(g, op, Sg, X, ep.xn, ep.xcol, Yc, B, shape, s); break;
+    case 4: cell_gemm_p<T, TV, 4>(g, op, Sg, X, ep.xn, ep.xcol, Yc, B, shape, s); break;
+    case 5: cell_gemm_p<T, TV, 5>(g, op, Sg, X, ep.xn, ep.xcol, Yc, B, shape, s); break;
     default: throw Error(IDFT_EINVAL, "degree not supported by the HIP kernels (2..5)");
   }
   IDFT_HIP(hipGetLastError());
@@ -1504,15 +1834,16 @@ int launch_cell_apply(const Geo& g, const T* H, const T* Sg, const TV* S, const 
   IDFT_HIP(hipGetLastError());
   return (int)(gx * gy);
 }
-template int launch_cell_apply<float, float>(const Geo&, const float*, const float*, const float*,
-                                             const float*, float*, int,
+template int launch_cell_apply<float, float>(const Geo&, const CellOp<float>&, const float*,
+                                             const float*, const float*, float*, int,
                                              const ApplyEpilogue<float>&, const GemmShape&,
                                              hipStream_t);
-template int launch_cell_apply<double, double>(const Geo&, const double*, const double*,
-                                               const double*, const double*, double*, int,
+template int launch_cell_apply<double, double>(const Geo&, const CellOp<double>&,
+                                               const double*, const double*, const double*,
+                                               double*, int,
                                                const ApplyEpilogue<double>&, const GemmShape&,
                                                hipStream_t);
-template int launch_cell_apply<float, double>(const Geo&, const float*, const float*,
+template int launch_cell_apply<float, double>(const Geo&, const CellOp<float>&, const float*,
                                               const double*, const double*, float*, int,
                                               const ApplyEpilogue<double>&, const GemmShape&,
                                               hipStream_t);

@@ -210,14 +210,41 @@ def test_every_tile_shape_keeps_parity(glib, olib, p, ncols):
     for cp in (1, 2, 4, 8):
         for vt in (16, 32, 64, 80):
             for prec, tol in ((abi.FP64, 1e-13), (abi.MIXED, 2e-6)):
-                for tp in (0, 1):  # tile pass applies to 16 (FP64) / 32 (FP32) columns
-                    g.set_apply_shape(prec, cp, vt, tile_pass=tp)
-                    assert rel(g.apply_hamiltonian(x, precision=prec), yo) < tol, (cp, vt, tp, prec)
+                # the tile pass applies to 16 (FP64) / 32 (FP32) columns
+                for k in (abi.KERNEL_ROW_PASS, abi.KERNEL_TILE_PASS):
+                    g.set_apply_shape(prec, cp, vt, kernel=k)
+                    assert rel(g.apply_hamiltonian(x, precision=prec), yo) < tol, (cp, vt, k, prec)
     for gr in (1, 2, 3, 7, 4096):
         for prec, tol in ((abi.FP64, 1e-13), (abi.MIXED, 2e-6)):
             g.set_apply_shape(prec, 1, 16, gr)
             assert rel(g.apply_hamiltonian(x, precision=prec), yo) < tol, (gr, prec)
     best = g.autotune_apply(ncols, abi.FP64, reps=2)
-    assert best["cells_per_wg"] in (1, 2, 4, 8) and best["vectors_per_tile"] % 16 == 0
-    assert best["gather_rows"] in (0, 1, 2, 4, 8) and best["tile_pass"] in (0, 1)
+    assert best["gather_rows"] in (0, 1, 2, 4, 8) and best["kernel"] in (0, 1, 2)
+    if best["kernel"] == abi.KERNEL_SUM_FACTOR:
+        assert best["vectors_per_tile"] in (2, 4, 8)
+    else:
+        assert best["cells_per_wg"] in (1, 2, 4, 8) and best["vectors_per_tile"] % 16 == 0
     assert rel(g.apply_hamiltonian(x), yo) < 1e-13
+
+
+@pytest.mark.parametrize("p", [2, 3, 4, 5])
+@pytest.mark.parametrize("ncols", [1, 3, 12, 20, 80])
+def test_sum_factorised_cell_product(glib, olib, p, ncols):
+    """The sum-factorised cell product (no stored H_e: 1D contractions with the
+    per-cell quadrature weights) against the oracle's dense cell matrices, at
+    every column count per work-group and per item, FP64 and FP32; ragged last
+    column groups included (3, 20)."""
+    _say("sum_factor", p, ncols)
+    g, o, s = bare_pair(glib, olib, 3, p, 4, precision=abi.MIXED)
+    x = np.random.default_rng(11 * p + ncols).standard_normal((s.n_nodes, ncols))
+    yo = o.apply_hamiltonian(x)
+    for cg in (2, 4, 8, 16):
+        for vc in (1, 2, 4):  # columns per item (the cells_per_wg argument)
+            for prec, tol in ((abi.FP64, 1e-13), (abi.MIXED, 2e-6)):
+                fp64 = prec == abi.FP64
+                if vc > cg or (fp64 and (cg == 16 or vc == 4)):
+                    with pytest.raises(RuntimeError):
+                        g.set_apply_shape(prec, vc, cg, kernel=abi.KERNEL_SUM_FACTOR)
+                    continue
+                g.set_apply_shape(prec, vc, cg, kernel=abi.KERNEL_SUM_FACTOR)
+                assert rel(g.apply_hamiltonian(x, precision=prec), yo) < tol, (cg, vc, prec)
