@@ -254,6 +254,9 @@ def main():
     ap.add_argument("--shape", type=int, nargs=4, default=[1, 16, 0, 0],
                     metavar=("CELLS_PER_WG", "VECTORS_PER_TILE", "GATHER_ROWS", "TILE_PASS"))
     ap.add_argument("--reps", type=int, default=5)
+    ap.add_argument("--relax-ratio", type=float, default=None,
+                    help="MIXED MINRES: residual / tolerance ratio below which the FP32 cell "
+                         "operator takes over (library default if omitted)")
     ap.add_argument("--batch-count", type=int, default=96,
                     help="problems of --config uneven_batch")
     ap.add_argument("--batch-steps", type=int, default=10)
@@ -285,6 +288,8 @@ def main():
     tol_kw = dict(cheb_tol=args.cheb_tol, initial_tol_cheb=args.cheb_tol)
     if args.no_graph:
         tol_kw["minres_graph"] = 0
+    if args.relax_ratio is not None:
+        tol_kw["relax_ratio"] = args.relax_ratio
 
     log("mixed precision run")
     cm, mixed = run_bfgs(lib, prob, abi.MIXED, args.warmup, args.steps, tol_kw)

@@ -33,27 +33,31 @@ int idft_profile_apply(idft_ctx* ctx, int32_t ncols, int32_t precision, int32_t 
                                     quadrature weights (sum factorisation) */
 
 /* Sets the operator launch shape used at the given precision: the
- * cell-product kernel and its shape, and the node rows per node-gather
- * work-group (gather_rows in 1..4096, or 0 for the default of about 16384
- * gather work-groups).
+ * cell-product kernel and its shape, and the node chunks per node-gather
+ * work-group (gather_rows in 1..4096 chunks of 256 / (columns per thread
+ * group) consecutive nodes, or 0 for the default of about 16384 gather
+ * work-groups).
  *   IDFT_KERNEL_ROW_PASS / IDFT_KERNEL_TILE_PASS: cells_per_wg in 1..64,
  *     vectors_per_tile a multiple of 16 in 16..80; the tile pass keeps
  *     partial sums of all block rows per wave and applies to 16-column tiles
  *     (and 32 in FP32), wider tiles run the row pass.  The stored H_e is
  *     built on first use.
  *   IDFT_KERNEL_SUM_FACTOR: vectors_per_tile = columns per work-group, 2, 4
- *     or 8 (FP64) and also 16 (FP32); cells_per_wg in 1..64, unused. */
+ *     or 8 (FP64) and also 16 (FP32); cells_per_wg = columns per work item,
+ *     1 or 2 (FP64) and also 4 (FP32), at most vectors_per_tile. */
 int idft_set_apply_shape(idft_ctx* ctx, int32_t precision, int32_t cells_per_wg,
                          int32_t vectors_per_tile, int32_t kernel, int32_t gather_rows);
 
 /* Autotuner of the cell operator: times the sum-factorised kernel at every
- * supported column count per work-group, then every stored-H shape
+ * supported column count per work-group and per item, then every stored-H shape
  * cells-per-work-group in {1,2,4,8} x vectors-per-tile in {16,32,64} (tiles
  * wider than the block collapse to one pass) x {row pass, tile pass} where
- * it applies, then node rows per gather work-group in {default,1,2,4,8} with
- * the best, with idft_profile_apply on ncols columns at the given precision,
- * keeps the fastest for this context and precision, reports it, and frees
- * the stored H_e when no precision's kernel reads it.  The kernels sum in
+ * it applies, then node chunks per gather work-group in {default,1,2,4,8} with
+ * the best.  Each shape is scored on ncols columns at the given precision by
+ * the device time of a plain application (as idft_profile_apply) plus that of
+ * one whose input carries node and column scales (as in the MINRES steps);
+ * the fastest is kept for this context and precision and reported (ms_best:
+ * that sum), and the stored H_e is freed when no precision's kernel reads it.  The kernels sum in
  * different orders, so results agree to rounding, not bit for bit, across
  * kernels.  No counterpart in the reference (its cell matrices go through a
  * library batched GEMM). */

@@ -151,7 +151,8 @@ struct GemmShape {
                               // sum-factorised kernel: columns per item (1, 2, 4)
   int vectors_per_tile = 80;  // stored-H kernels: columns per tile (rounded to 16; >= B: one
                               // pass); sum-factorised kernel: columns per work-group (2, 4, 8, 16)
-  int gather_rows = 0;        // node rows per gather work-group (0: about kGatherBlocks groups)
+  int gather_rows = 0;        // node chunks (256 / (B / VW) nodes each) per gather work-group
+                              // (0: about kGatherBlocks groups)
   int kernel = CK_ROW_PASS;   // CellKernel; the tile pass applies to 16-column tiles (FP32 also 32)
 };
 // The cell operator at one precision: the stored upper tiles of H_e (row and

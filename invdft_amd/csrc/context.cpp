@@ -229,6 +229,64 @@ struct idft_ctx {
     H32.release();
     storedH = false;
   }
+  // Device ms per operator application on ncols random columns (HIP events
+  // on the context stream, 2 warm-up + reps applications), FP32 or FP64
+  // operator.  scaled: the input carries a node and a column scale, as in the
+  // MINRES steps (FP32 operator on FP64 vectors there); ms_gemm (optional):
+  // the cell-product kernel alone, unscaled.
+  double profile_ms(int ncols, bool f32, int reps, bool scaled, double* ms_gemm) {
+    const size_t n = NB(ncols);
+    std::vector<double> h(n);
+    uint64_t sd = 12345;
+    for (auto& v : h) v = rand_pm1(sd);
+    DevBuf<double> xd, yd(n), ones;
+    xd.upload(h.data(), n, s);
+    const std::vector<double> one(ncols, 1.0);
+    ones.upload(one.data(), one.size(), s);
+    DevBuf<float> xf, yf;
+    if (f32 && !scaled) {
+      xf.resize(n);
+      yf.resize(n);
+      launch_convert_d2f((int64_t)n, xf.p, xd.p, s);
+    }
+    auto once = [&] {
+      if (scaled) {
+        ApplyEpilogue<double> ep;
+        ep.Y = yd.p;
+        ep.xn = Djac.p;
+        ep.xcol = ones.p;
+        if (f32) apply_f32op(xd.p, ncols, ep);
+        else apply<double>(xd.p, ncols, ep);
+      } else if (f32) {
+        happly32(xf.p, yf.p, ncols, 1.0);
+      } else {
+        happly64(xd.p, yd.p, ncols, 1.0);
+      }
+    };
+    auto gemm = [&] {
+      if (f32) launch_cell_gemm<float>(g, op32(), S32.p, xf.p, nullptr, Yc32.p, ncols, shape32, s);
+      else launch_cell_gemm<double>(g, op64(), S.p, xd.p, nullptr, Yc64.p, ncols, shape64, s);
+    };
+    hipEvent_t e0, e1;
+    IDFT_HIP(hipEventCreate(&e0));
+    IDFT_HIP(hipEventCreate(&e1));
+    auto timed = [&](auto&& fn) {
+      fn();
+      fn();
+      IDFT_HIP(hipEventRecord(e0, s));
+      for (int r = 0; r < reps; ++r) fn();
+      IDFT_HIP(hipEventRecord(e1, s));
+      IDFT_HIP(hipEventSynchronize(e1));
+      float el = 0.0f;
+      IDFT_HIP(hipEventElapsedTime(&el, e0, e1));
+      return (double)el / reps;
+    };
+    const double ms = timed(once);
+    if (ms_gemm && !scaled) *ms_gemm = timed(gemm);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return ms;
+  }
   // The sum-factorised kernel's compile-time 1D tables against this
   // context's FE tables (fe.cpp): they must agree to rounding.
   void check_sf_tables() const {
@@ -1397,46 +1455,7 @@ int idft_profile_apply(idft_ctx* c, int32_t ncols, int32_t prec, int32_t reps, d
                        double* ms_gemm) {
   if (!c || !ms || ncols < 1 || ncols > kMaxCols || reps < 1) return IDFT_EINVAL;
   if (!c->haveH) { c->err = "no Hamiltonian"; return IDFT_ESTATE; }
-  return guarded(c, [&] {
-    const size_t n = c->NB(ncols);
-    std::vector<double> h(n);
-    uint64_t sd = 12345;
-    for (auto& v : h) v = rand_pm1(sd);
-    DevBuf<double> xd, yd(n);
-    xd.upload(h.data(), n, c->s);
-    DevBuf<float> xf, yf;
-    const bool f32 = prec == IDFT_MIXED;
-    if (f32) {
-      xf.resize(n); yf.resize(n);
-      launch_convert_d2f((int64_t)n, xf.p, xd.p, c->s);
-    }
-    auto once = [&] {
-      if (f32) c->happly32(xf.p, yf.p, ncols, 1.0);
-      else c->happly64(xd.p, yd.p, ncols, 1.0);
-    };
-    auto gemm = [&] {
-      if (f32) launch_cell_gemm<float>(c->g, c->op32(), c->S32.p, xf.p, nullptr, c->Yc32.p, ncols, c->shape32, c->s);
-      else launch_cell_gemm<double>(c->g, c->op64(), c->S.p, xd.p, nullptr, c->Yc64.p, ncols, c->shape64, c->s);
-    };
-    hipEvent_t e0, e1;
-    IDFT_HIP(hipEventCreate(&e0));
-    IDFT_HIP(hipEventCreate(&e1));
-    auto timed = [&](auto&& fn) {
-      fn();
-      fn();
-      IDFT_HIP(hipEventRecord(e0, c->s));
-      for (int r = 0; r < reps; ++r) fn();
-      IDFT_HIP(hipEventRecord(e1, c->s));
-      IDFT_HIP(hipEventSynchronize(e1));
-      float el = 0.0f;
-      IDFT_HIP(hipEventElapsedTime(&el, e0, e1));
-      return (double)el / reps;
-    };
-    *ms = timed(once);
-    if (ms_gemm) *ms_gemm = timed(gemm);
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-  });
+  return guarded(c, [&] { *ms = c->profile_ms(ncols, prec == IDFT_MIXED, reps, false, ms_gemm); });
 }
 
 int idft_set_vxc_output(idft_ctx* c, const char* folder, const char* postfix, int32_t frequency) {
@@ -1490,10 +1509,17 @@ int idft_autotune_apply(idft_ctx* c, int32_t ncols, int32_t prec, int32_t reps,
       const int e = std::min(vt, vmax);
       if (std::find(widths.begin(), widths.end(), e) == widths.end()) widths.push_back(e);
     }
+    // score: a plain application (eigensolver) plus one with node and
+    // column scales on the input (MINRES steps)
     auto trial = [&] {
       double ms = 0.0;
-      int rc = idft_profile_apply(c, ncols, prec, reps, &ms, nullptr);
-      if (rc) { target = saved; throw Error(rc, c->err); }
+      try {
+        ms = c->profile_ms(ncols, f32, reps, false, nullptr) +
+             c->profile_ms(ncols, f32, reps, true, nullptr);
+      } catch (...) {
+        target = saved;
+        throw;
+      }
       if (ms < bestMs) { bestMs = ms; best = target; }
     };
     target.gather_rows = 0;

@@ -487,7 +487,7 @@ __global__ __launch_bounds__(kSymmThreads) void k_cell_symm(Geo g, const T* __re
 template <typename T, typename TV>
 __device__ inline T cell_input(const T* __restrict__ S, const TV* __restrict__ X,
                                const double* __restrict__ xn, const double* __restrict__ xcol,
-                               int64_t n, int B, int col) {
+                               uint32_t n, uint32_t B, uint32_t col) {
   if constexpr (std::is_same<T, TV>::value) {
     T v = S[n] * X[n * B + col];
     if (xn) v *= (T)xn[n];
@@ -542,14 +542,22 @@ __global__ __launch_bounds__(kThreads) void k_cell_sf(Geo g, T hq, const T* __re
   typedef Pack<T, VC> pk;
   __shared__ pk r1[SZ1 * NV];
   __shared__ pk r2[SZ2 * NV];
-  // cells dealt to the 8 XCDs in contiguous runs
-  const int64_t per = (g.ncell + 7) / 8;
-  const int64_t e = (blockIdx.x % 8) * per + blockIdx.x / 8;
-  if (e >= g.ncell) return;
-  const int c0 = blockIdx.y * CG;
+  // (cell, column group) pairs dealt to the 8 XCDs in contiguous runs, the
+  // groups of one cell at consecutive slots of one XCD, so that they share
+  // that XCD's L2 for the cell's rows of X and of the cell scratch (each group
+  // holds CG of the B columns of a row).  Every index below fits in 32 bits
+  // (the host checks N B and cells n_e B against 2^32).
+  const uint32_t ncell = (uint32_t)g.ncell, nc = (uint32_t)g.nc, np = (uint32_t)g.p;
+  const uint32_t NN = (uint32_t)g.N1, ub = (uint32_t)B;
+  const uint32_t ngrp = (ub + CG - 1) / CG, npair = ncell * ngrp;
+  const uint32_t per = (npair + 7) / 8;
+  const uint32_t k = (blockIdx.x % 8) * per + blockIdx.x / 8;
+  if (k >= npair) return;
+  const uint32_t e = k / ngrp;
+  const int c0 = (int)(k % ngrp) * CG;
   const int bw = min(CG, B - c0);
   const int tid = threadIdx.x;
-  const int64_t cx = e % g.nc, cy = (e / g.nc) % g.nc, cz = e / ((int64_t)g.nc * g.nc);
+  const uint32_t cx = e % nc, cy = (e / nc) % nc, cz = e / (nc * nc);
   pk* t1 = r1;                   // [N2][QP]
   pk* ka = r1 + N2 * QP * NV;    // [N2][N1P]
   pk* kb = ka + N2 * N1P * NV;   // [N2][N1P]
@@ -559,19 +567,19 @@ __global__ __launch_bounds__(kThreads) void k_cell_sf(Geo g, T hq, const T* __re
   pk* s2 = r1;                   // [N1][Q][Q]
   pk* yk = r1 + N1 * Q2 * NV;    // [N2][N1P]
   pk* s1 = r2;                   // [N2][QP]
-  const T* we = W + e * Q3;
+  const T* we = W + e * (uint32_t)Q3;
   // stage 1: x lines ln = c*N1 + b
   for (int it = tid; it < N2 * NV; it += kThreads) {
     const int jv = it % NV, ln = it / NV;
     pk v[N1];
-    const int b = ln % N1, c = ln / N1;
-    const int64_t n0 = ((cz * g.p + c) * g.N1 + (cy * g.p + b)) * (int64_t)g.N1 + cx * g.p;
+    const uint32_t b = ln % N1, c = ln / N1;
+    const uint32_t n0 = ((cz * np + c) * NN + (cy * np + b)) * NN + cx * np;
 #pragma unroll
     for (int a = 0; a < N1; ++a)
 #pragma unroll
       for (int u = 0; u < VC; ++u) {
         const int j = jv * VC + u;
-        v[a].v[u] = j < bw ? cell_input<T, TV>(S, X, xn, xcol, n0 + a, B, c0 + j) : T(0);
+        v[a].v[u] = j < bw ? cell_input<T, TV>(S, X, xn, xcol, n0 + a, ub, c0 + j) : T(0);
       }
     static_for<0, Q>([&](auto q) {
       pk s;
@@ -708,7 +716,7 @@ __global__ __launch_bounds__(kThreads) void k_cell_sf(Geo g, T hq, const T* __re
     pk v[Q];
 #pragma unroll
     for (int qx = 0; qx < Q; ++qx) v[qx] = s1[(ln * QP + qx) * NV + jv];
-    T* yc = Yc + ((size_t)e * NE + ln * N1) * B + c0 + jv * VC;
+    const uint32_t y0 = (e * NE + ln * N1) * ub + c0 + jv * VC;
     static_for<0, N1>([&](auto a) {
       const pk kin = yk[(ln * N1P + IDFT_IC(a)) * NV + jv];
       pk s;
@@ -719,7 +727,7 @@ __global__ __launch_bounds__(kThreads) void k_cell_sf(Geo g, T hq, const T* __re
       });
 #pragma unroll
       for (int u = 0; u < VC; ++u)
-        if (jv * VC + u < bw) yc[(size_t)IDFT_IC(a) * B + u] = s.v[u];
+        if (jv * VC + u < bw) Yc[y0 + IDFT_IC(a) * ub + u] = s.v[u];
     });
   }
 }
@@ -731,10 +739,11 @@ __global__ __launch_bounds__(kThreads) void k_cell_sf(Geo g, T hq, const T* __re
 //   Y = beta Y + alpha S[n] sum + a ca[j] xn[n] X + b cb[j] Z + c cc[j] W
 // (xn, the operator input's node scale, optional).  With `partial`, the
 // per-column block partials of V.Y (FP64) for the scaled operator input
-// V = xcol[j] xn[n] Xin, optionally stored.  Block (bx, by) covers node rows
-// (iy, iz) in [by * R, by * R + R) and x positions [bx * per, bx * per + per)
-// of each row; thread t owns column t % B and x offset t / B for its whole
-// life.
+// V = xcol[j] xn[n] Xin, optionally stored.  Nodes are taken in their
+// linear order n = (iz N1 + iy) N1 + ix: block b covers R consecutive chunks
+// of per = 256 / (B / VW) nodes, [(b R + r) per, (b R + r + 1) per), so a
+// chunk may cross x rows and every lane has work; thread t owns column group
+// t % (B / VW) and position t / (B / VW) within each chunk.
 // VW consecutive elements of a node-major row (16-byte aligned for VW > 1).
 template <typename T, int VW>
 __device__ inline void vload(const T* __restrict__ p, T (&o)[VW]) {
@@ -776,11 +785,10 @@ __global__ __launch_bounds__(kThreads) void k_cell_gather(
   const uint32_t nb = B / VW;  // column groups of a node
   const uint32_t tid = threadIdx.x, per = kThreads / nb;
   const uint32_t jg = tid % nb, off = tid / nb, j0 = jg * VW;
-  const uint32_t ix = blockIdx.x * per + off;
   double dot[VW];
 #pragma unroll
   for (int v = 0; v < VW; ++v) dot[v] = 0.0;
-  if (off < per && ix < N1) {
+  if (off < per) {
     double caj[VW], cbj[VW], ccj[VW], xs[VW];
 #pragma unroll
     for (int v = 0; v < VW; ++v) {
@@ -789,19 +797,22 @@ __global__ __launch_bounds__(kThreads) void k_cell_gather(
       ccj[v] = W ? c * (cc ? cc[j0 + v] : 1.0) : 0.0;
       xs[v] = xcol ? xcol[j0 + v] : 1.0;
     }
-    // x contributions (cell, local index) of this thread's column of nodes
-    uint32_t cxs[2], axs[2];
-    int nx = 0;
-    {
-      const uint32_t q = ix / P, r = ix - q * P;
-      if (r == 0) {
-        if (q > 0) { cxs[nx] = q - 1; axs[nx++] = P; }
-        if (q < nc) { cxs[nx] = q; axs[nx++] = 0; }
-      } else { cxs[nx] = q; axs[nx++] = r; }
-    }
-    const uint32_t row0 = blockIdx.y * R, row1 = min(row0 + R, N1 * N1);
-    for (uint32_t row = row0; row < row1; ++row) {
+    const uint32_t nn = N1 * N1 * N1;
+    for (uint32_t rr = 0; rr < R; ++rr) {
+      const uint32_t nidx = (blockIdx.x * R + rr) * per + off;
+      if (nidx >= nn) break;
+      const uint32_t ix = nidx % N1, row = nidx / N1;
       const uint32_t iy = row % N1, iz = row / N1;
+      // contributions (cell, local index) along each axis
+      uint32_t cxs[2], axs[2];
+      int nx = 0;
+      {
+        const uint32_t q = ix / P, r = ix - q * P;
+        if (r == 0) {
+          if (q > 0) { cxs[nx] = q - 1; axs[nx++] = P; }
+          if (q < nc) { cxs[nx] = q; axs[nx++] = 0; }
+        } else { cxs[nx] = q; axs[nx++] = r; }
+      }
       uint32_t cys[2], ays[2], czs[2], azs[2];
       int ny = 0, nz = 0;
       {
@@ -1670,7 +1681,11 @@ template <typename T, typename TV, int P, int CG>
 static void cell_sf_cg(const Geo& g, const CellOp<T>& op, const T* S, const TV* X,
                        const double* xn, const double* xcol, T* Yc, int B, int vc,
                        hipStream_t s) {
-  dim3 grid((unsigned)(8 * ((g.ncell + 7) / 8)), (unsigned)((B + CG - 1) / CG)), block(kThreads);
+  check_u32(g.N * B);
+  check_u32(cell_scratch_elems(g, B));
+  const int64_t npair = g.ncell * ((B + CG - 1) / CG);
+  check_u32(8 * ((npair + 7) / 8));
+  dim3 grid((unsigned)(8 * ((npair + 7) / 8))), block(kThreads);
   const T hq = (T)op.kin_scale;
   if (vc == 1) {
     k_cell_sf<T, TV, P, CG, 1><<<grid, block, 0, s>>>(g, hq, op.W, S, X, xn, xcol, Yc, B);
@@ -1755,18 +1770,18 @@ static int gather_vw(int B, int vbytes) {
   return B % 2 == 0 ? 2 : 1;
 }
 
-// Gather grid: x chunks of kThreads / (B / VW) nodes times groups of R node
-// rows, with R = `rows_req` if positive, else chosen so that the grid has at
-// most about kGatherBlocks blocks.
+// Gather grid: blocks of R chunks of kThreads / (B / VW) nodes each, with
+// R = `rows_req` if positive, else chosen so that the grid has at most about
+// kGatherBlocks blocks.
 static void gather_grid(const Geo& g, int B, int vw, int rows_req, unsigned& gx, unsigned& gy,
                         uint32_t& R) {
   const int per = kThreads / (B / vw);
-  gx = (unsigned)((g.N1 + per - 1) / per);
-  const int64_t rows = (int64_t)g.N1 * g.N1;
-  int64_t r = rows_req > 0 ? rows_req : (rows * gx + kGatherBlocks - 1) / kGatherBlocks;
+  const int64_t chunks = (g.N + per - 1) / per;
+  int64_t r = rows_req > 0 ? rows_req : (chunks + kGatherBlocks - 1) / kGatherBlocks;
   if (r < 1) r = 1;
   R = (uint32_t)r;
-  gy = (unsigned)((rows + r - 1) / r);
+  gx = (unsigned)((chunks + r - 1) / r);
+  gy = 1;
 }
 
 int gather_blocks(const Geo& g, int B, int rows, int vbytes) {
