@@ -45,11 +45,12 @@ def sf_flops_per_cell_column(p):
            + n1 ** 3 * q)                    # x lines back
     return 2 * fma + q ** 3 + n1 ** 3
 # HBM bytes per launch of the dominant kernel from two rocprofv3 --pmc runs
-# (FETCH_SIZE, WRITE_SIZE) of `bench.py --kernels-only --shape 2 16 4 0` /
-# `... 1 16 0 1`, FETCH_SIZE scaled by the copy calibration (x1.926), WRITE_SIZE
-# as read; the summary with the commands is
-# profiles/cell_gemm_inverse_24c_p5_pmc.md.
+# (FETCH_SIZE, WRITE_SIZE) of `bench.py --kernels-only --shape ...`, FETCH_SIZE
+# scaled by the copy calibration (x1.926), WRITE_SIZE as read; the summaries
+# with the commands are profiles/cell_sf_inverse_24c_p5_pmc.md (shape 1 4 4 2)
+# and profiles/cell_gemm_inverse_24c_p5_pmc.md (2 16 4 0 and 1 16 0 1).
 MEASURED_TRAFFIC = {
+    ("inverse_24c_p5", "k_cell_sf<float,float,5,4,1>"): 276.3e6,
     ("inverse_24c_p5", "k_cell_gemm<float,float,5,1>"): 2613e6,
     ("inverse_24c_p5", "k_cell_symm<float,float,5,1>"): 1691e6,
 }
@@ -62,9 +63,11 @@ def log(*a):
 def run_bfgs(lib, prob, precision, warmup, steps, tol_kw):
     from invdft_amd import abi
     ctx = prob.context(lib, precision=precision, **tol_kw)
-    ctx.set_potential(prob.vxc0)  # cell matrices for the autotuner (untimed)
+    ctx.set_potential(prob.vxc0)  # cell operator for the autotuner (untimed)
+    log("  autotuning the FP64 cell operator")
     tuned = {"fp64": ctx.autotune_apply(prob.desc.n_eig, abi.FP64)}
     if precision == abi.MIXED:
+        log("  autotuning the FP32 cell operator")
         tuned["fp32"] = ctx.autotune_apply(prob.desc.n_eig, abi.MIXED)
     log(f"  autotuned cell-product tile shapes: {tuned}")
     t0 = time.perf_counter()

@@ -19,7 +19,8 @@ int idft_hip_init(void);
 /* Average device time (HIP events on the context's stream) of one
  * application of the Loewdin cell operator on a block of ncols random
  * columns, FP64 or FP32 (precision = IDFT_FP64 / IDFT_MIXED), over reps
- * repetitions after 2 warm-up applications: ms_per_apply for the whole
+ * repetitions after 2 warm-up applications, taking three different random
+ * blocks in turn so that the input is not cache-resident: ms_per_apply for the whole
  * operator (MFMA cell-product kernel + node-gather kernel), ms_gemm (may be
  * NULL) for the MFMA cell-product kernel alone.  No counterpart in the
  * reference; used by the benchmark to measure the dominant kernel live. */
@@ -49,7 +50,8 @@ int idft_set_apply_shape(idft_ctx* ctx, int32_t precision, int32_t cells_per_wg,
                          int32_t vectors_per_tile, int32_t kernel, int32_t gather_rows);
 
 /* Autotuner of the cell operator: times the sum-factorised kernel at every
- * supported column count per work-group and per item, then every stored-H shape
+ * supported column count per work-group (2 only for blocks narrower than 8
+ * columns) and per item, then every stored-H shape
  * cells-per-work-group in {1,2,4,8} x vectors-per-tile in {16,32,64} (tiles
  * wider than the block collapse to one pass) x {row pass, tile pass} where
  * it applies, then node chunks per gather work-group in {default,1,2,4,8} with

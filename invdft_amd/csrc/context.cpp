@@ -234,38 +234,64 @@ struct idft_ctx {
   // operator.  scaled: the input carries a node and a column scale, as in the
   // MINRES steps (FP32 operator on FP64 vectors there); ms_gemm (optional):
   // the cell-product kernel alone, unscaled.
+  // Random input blocks of the profiling calls, made once per width and kept
+  // while the autotuner times its shapes: kProfBlocks different blocks taken
+  // in turn, so that an application does not find its input in the caches
+  // (as in the solvers, where other vector kernels run between applications).
+  DevBuf<double> profX, profY, profOnes;
+  DevBuf<float> profXf, profYf;
+  int profCols = 0;
+  void release_profile_block() {
+    profX.release();
+    profY.release();
+    profOnes.release();
+    profXf.release();
+    profYf.release();
+    profCols = 0;
+  }
   double profile_ms(int ncols, bool f32, int reps, bool scaled, double* ms_gemm) {
+    constexpr int kProfBlocks = 3;
     const size_t n = NB(ncols);
-    std::vector<double> h(n);
-    uint64_t sd = 12345;
-    for (auto& v : h) v = rand_pm1(sd);
-    DevBuf<double> xd, yd(n), ones;
-    xd.upload(h.data(), n, s);
-    const std::vector<double> one(ncols, 1.0);
-    ones.upload(one.data(), one.size(), s);
-    DevBuf<float> xf, yf;
-    if (f32 && !scaled) {
-      xf.resize(n);
-      yf.resize(n);
-      launch_convert_d2f((int64_t)n, xf.p, xd.p, s);
+    if (profCols != ncols) {
+      std::vector<double> h(n * kProfBlocks);
+      uint64_t sd = 12345;
+      for (auto& v : h) v = rand_pm1(sd);
+      profX.upload(h.data(), h.size(), s);
+      profY.resize(n);
+      const std::vector<double> one(ncols, 1.0);
+      profOnes.upload(one.data(), one.size(), s);
+      profXf.release();
+      profYf.release();
+      profCols = ncols;
     }
+    if (f32 && !scaled && !profXf.p) {
+      profXf.resize(n * kProfBlocks);
+      profYf.resize(n);
+      launch_convert_d2f((int64_t)(n * kProfBlocks), profXf.p, profX.p, s);
+    }
+    int blk = 0;  // input block of the next application
+    auto xdp = [&] { return profX.p + n * (blk++ % kProfBlocks); };
+    auto xfp = [&] { return profXf.p + n * (blk++ % kProfBlocks); };
+    DevBuf<double>& yd = profY;
+    DevBuf<double>& ones = profOnes;
+    DevBuf<float>& yf = profYf;
     auto once = [&] {
       if (scaled) {
         ApplyEpilogue<double> ep;
         ep.Y = yd.p;
         ep.xn = Djac.p;
         ep.xcol = ones.p;
-        if (f32) apply_f32op(xd.p, ncols, ep);
-        else apply<double>(xd.p, ncols, ep);
+        if (f32) apply_f32op(xdp(), ncols, ep);
+        else apply<double>(xdp(), ncols, ep);
       } else if (f32) {
-        happly32(xf.p, yf.p, ncols, 1.0);
+        happly32(xfp(), yf.p, ncols, 1.0);
       } else {
-        happly64(xd.p, yd.p, ncols, 1.0);
+        happly64(xdp(), yd.p, ncols, 1.0);
       }
     };
     auto gemm = [&] {
-      if (f32) launch_cell_gemm<float>(g, op32(), S32.p, xf.p, nullptr, Yc32.p, ncols, shape32, s);
-      else launch_cell_gemm<double>(g, op64(), S.p, xd.p, nullptr, Yc64.p, ncols, shape64, s);
+      if (f32) launch_cell_gemm<float>(g, op32(), S32.p, xfp(), nullptr, Yc32.p, ncols, shape32, s);
+      else launch_cell_gemm<double>(g, op64(), S.p, xdp(), nullptr, Yc64.p, ncols, shape64, s);
     };
     hipEvent_t e0, e1;
     IDFT_HIP(hipEventCreate(&e0));
@@ -1455,7 +1481,10 @@ int idft_profile_apply(idft_ctx* c, int32_t ncols, int32_t prec, int32_t reps, d
                        double* ms_gemm) {
   if (!c || !ms || ncols < 1 || ncols > kMaxCols || reps < 1) return IDFT_EINVAL;
   if (!c->haveH) { c->err = "no Hamiltonian"; return IDFT_ESTATE; }
-  return guarded(c, [&] { *ms = c->profile_ms(ncols, prec == IDFT_MIXED, reps, false, ms_gemm); });
+  return guarded(c, [&] {
+    *ms = c->profile_ms(ncols, prec == IDFT_MIXED, reps, false, ms_gemm);
+    c->release_profile_block();
+  });
 }
 
 int idft_set_vxc_output(idft_ctx* c, const char* folder, const char* postfix, int32_t frequency) {
@@ -1526,6 +1555,11 @@ int idft_autotune_apply(idft_ctx* c, int32_t ncols, int32_t prec, int32_t reps,
     target.kernel = CK_SUM_FACTOR;
     for (int cg : {2, 4, 8, 16}) {
       if (cg > 2 && cg / 2 >= ncols) continue;  // mostly empty column groups
+      // 2 columns per work-group gather each node row of the block once per
+      // 2 columns.  Between the solvers' other vector kernels this measured
+      // 1.3x slower than 4 columns at B = 12 (FP64 MINRES, kernel trace), a
+      // cost that a timing loop of back-to-back applications does not show.
+      if (cg == 2 && ncols >= 8) continue;
       for (int vc : {1, 2, 4}) {
         if (!sf_vc_supported(cg, vc, !f32)) continue;
         target.vectors_per_tile = cg;
@@ -1551,6 +1585,7 @@ int idft_autotune_apply(idft_ctx* c, int32_t ncols, int32_t prec, int32_t reps,
     }
     target = best;
     c->release_unused_h();
+    c->release_profile_block();
     if (cells_per_wg) *cells_per_wg = best.cells_per_wg;
     if (vectors_per_tile) *vectors_per_tile = best.vectors_per_tile;
     if (kernel) *kernel = best.kernel;
