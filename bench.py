@@ -229,7 +229,7 @@ def uneven_batch(lib, count, steps, workers):
     sm, sf = batch.summary(res_m, wall_m), batch.summary(res_f, wall_f)
     out = {"metric": f"problems/s, uneven batch of {count} inverse problems x {steps} BFGS steps",
            "value": sm["problems_per_s"], "unit": "problems/s", "n_gpus": 1,
-           "higher_is_better": True, "dtype": "f32-mfma+f64", "data": "synthetic",
+           "higher_is_better": True, "dtype": "f32+f64", "data": "synthetic",
            "config": {"workload": "uneven_batch", "problems": count, "bfgs_steps": steps,
                       "cells": "U{8..20}^3", "degree": "U{3,4,5}", "orbitals": "U{1..16}",
                       "workers": workers},
