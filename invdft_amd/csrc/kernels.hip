@@ -484,23 +484,6 @@ __global__ __launch_bounds__(kSymmThreads) void k_cell_symm(Geo g, const T* __re
 // The regions r1 / r2 are reused: (t1, ka, kb) -> (s2, yk) in r1 and
 // (t2, k1, k2) -> s1 in r2, each rewritten only after the barrier that ends
 // the stage reading it.
-template <typename T, typename TV>
-__device__ inline T cell_input(const T* __restrict__ S, const TV* __restrict__ X,
-                               const double* __restrict__ xn, const double* __restrict__ xcol,
-                               uint32_t n, uint32_t B, uint32_t col) {
-  if constexpr (std::is_same<T, TV>::value) {
-    T v = S[n] * X[n * B + col];
-    if (xn) v *= (T)xn[n];
-    if (xcol) v *= (T)xcol[col];
-    return v;
-  } else {  // wider input vectors: scale in FP64, round once
-    double w = (double)S[n] * (double)X[n * B + col];
-    if (xn) w *= xn[n];
-    if (xcol) w *= xcol[col];
-    return (T)w;
-  }
-}
-
 __host__ __device__ constexpr int cmax(int a, int b) { return a > b ? a : b; }
 
 template <typename T, int VC>
@@ -574,13 +557,35 @@ __global__ __launch_bounds__(kThreads) void k_cell_sf(Geo g, T hq, const T* __re
     pk v[N1];
     const uint32_t b = ln % N1, c = ln / N1;
     const uint32_t n0 = ((cz * np + c) * NN + (cy * np + b)) * NN + cx * np;
+    // the line's N1 nodes are consecutive: one base pointer per array, the
+    // nodes at immediate offsets (the X rows B apart)
+    const T* Sp = S + n0;
+    const double* xnp = xn ? xn + n0 : nullptr;
 #pragma unroll
-    for (int a = 0; a < N1; ++a)
+    for (int u = 0; u < VC; ++u) {
+      const int j = jv * VC + u;
+      if (j < bw) {
+        const TV* Xp = X + (size_t)n0 * ub + (uint32_t)(c0 + j);
+        const double xc = xcol ? xcol[c0 + j] : 1.0;
 #pragma unroll
-      for (int u = 0; u < VC; ++u) {
-        const int j = jv * VC + u;
-        v[a].v[u] = j < bw ? cell_input<T, TV>(S, X, xn, xcol, n0 + a, ub, c0 + j) : T(0);
+        for (int a = 0; a < N1; ++a) {
+          if constexpr (std::is_same<T, TV>::value) {
+            T x = Sp[a] * Xp[(size_t)a * ub];
+            if (xnp) x *= (T)xnp[a];
+            if (xcol) x *= (T)xc;
+            v[a].v[u] = x;
+          } else {  // wider input vectors: scale in FP64, round once
+            double x = (double)Sp[a] * (double)Xp[(size_t)a * ub];
+            if (xnp) x *= xnp[a];
+            if (xcol) x *= xc;
+            v[a].v[u] = (T)x;
+          }
+        }
+      } else {
+#pragma unroll
+        for (int a = 0; a < N1; ++a) v[a].v[u] = T(0);
       }
+    }
     static_for<0, Q>([&](auto q) {
       pk s;
       pack_zero(s);
@@ -716,7 +721,7 @@ __global__ __launch_bounds__(kThreads) void k_cell_sf(Geo g, T hq, const T* __re
     pk v[Q];
 #pragma unroll
     for (int qx = 0; qx < Q; ++qx) v[qx] = s1[(ln * QP + qx) * NV + jv];
-    const uint32_t y0 = (e * NE + ln * N1) * ub + c0 + jv * VC;
+    T* yp = Yc + (size_t)((e * NE + ln * N1) * ub + c0 + jv * VC);
     static_for<0, N1>([&](auto a) {
       const pk kin = yk[(ln * N1P + IDFT_IC(a)) * NV + jv];
       pk s;
@@ -727,7 +732,7 @@ __global__ __launch_bounds__(kThreads) void k_cell_sf(Geo g, T hq, const T* __re
       });
 #pragma unroll
       for (int u = 0; u < VC; ++u)
-        if (jv * VC + u < bw) Yc[y0 + IDFT_IC(a) * ub + u] = s.v[u];
+        if (jv * VC + u < bw) yp[(size_t)IDFT_IC(a) * ub + u] = s.v[u];
     });
   }
 }
