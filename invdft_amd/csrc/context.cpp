@@ -1137,9 +1137,10 @@ idft_ctx* create_ctx(const idft_desc* d, const double* pot_base_q, const double*
     c->dt.Wc.upload(t.Wc.data(), t.Wc.size(), s);
     c->check_sf_tables();
     // default cell product: sum-factorised (no stored H_e) for blocks of up
-    // to 32 columns, where it is the faster kernel; the row pass over stored
-    // tiles for wider blocks (idft_autotune_apply refines both)
-    if (d->n_eig <= 32) {
+    // to 32 columns, where it is the faster kernel, if its cell scratch fits
+    // 32-bit indexing; the row pass over stored tiles otherwise
+    // (idft_autotune_apply refines both)
+    if (d->n_eig <= 32 && cell_scratch_elems(c->g, d->n_eig) < (int64_t)0xffffffffLL) {
       for (GemmShape* sh : {&c->shape64, &c->shape32}) {
         sh->kernel = CK_SUM_FACTOR;
         sh->vectors_per_tile = 4;

@@ -749,6 +749,24 @@ __global__ __launch_bounds__(kThreads) void k_cell_sf(Geo g, T hq, const T* __re
 // of per = 256 / (B / VW) nodes, [(b R + r) per, (b R + r + 1) per), so a
 // chunk may cross x rows and every lane has work; thread t owns column group
 // t % (B / VW) and position t / (B / VW) within each chunk.
+// Unsigned 32-bit division by a run-time constant d >= 2 through a
+// multiplier computed on the host (Granlund-Montgomery): with
+// s = ceil(log2 d) and m = floor(2^32 (2^s - d) / d) + 1,
+// n / d = (t + ((n - t) >> 1)) >> (s - 1), t = mulhi(n, m), for every n.
+struct FastDiv {
+  uint32_t m, s;
+};
+inline FastDiv make_fastdiv(uint32_t d) {
+  uint32_t s = 0;
+  while ((uint64_t(1) << s) < d) ++s;
+  const uint64_t m = ((uint64_t(1) << 32) * ((uint64_t(1) << s) - d)) / d + 1;
+  return FastDiv{(uint32_t)m, s};
+}
+__device__ inline uint32_t fast_div(uint32_t n, FastDiv f) {
+  const uint32_t t = __umulhi(n, f.m);
+  return (t + ((n - t) >> 1)) >> (f.s - 1);
+}
+
 // VW consecutive elements of a node-major row (16-byte aligned for VW > 1).
 template <typename T, int VW>
 __device__ inline void vload(const T* __restrict__ p, T (&o)[VW]) {
@@ -778,7 +796,7 @@ __device__ inline void vstore(T* __restrict__ p, const T (&o)[VW]) {
 
 template <typename TY, typename T, int P, int VW>
 __global__ __launch_bounds__(kThreads) void k_cell_gather(
-    Geo g, uint32_t B, uint32_t R, const TY* __restrict__ Yc, const T* __restrict__ S,
+    Geo g, FastDiv divN1, uint32_t B, uint32_t R, const TY* __restrict__ Yc, const T* __restrict__ S,
     T* __restrict__ Y, double alpha, double beta, const T* __restrict__ X, double a,
     const double* __restrict__ ca, const T* __restrict__ Z, double b,
     const double* __restrict__ cb, const T* __restrict__ W, double c,
@@ -806,8 +824,8 @@ __global__ __launch_bounds__(kThreads) void k_cell_gather(
     for (uint32_t rr = 0; rr < R; ++rr) {
       const uint32_t nidx = (blockIdx.x * R + rr) * per + off;
       if (nidx >= nn) break;
-      const uint32_t ix = nidx % N1, row = nidx / N1;
-      const uint32_t iy = row % N1, iz = row / N1;
+      const uint32_t row = fast_div(nidx, divN1), ix = nidx - row * N1;
+      const uint32_t iz = fast_div(row, divN1), iy = row - iz * N1;
       // contributions (cell, local index) along each axis
       uint32_t cxs[2], axs[2];
       int nx = 0;
@@ -834,30 +852,33 @@ __global__ __launch_bounds__(kThreads) void k_cell_gather(
           if (q < nc) { czs[nz] = q; azs[nz++] = 0; }
         } else { czs[nz] = q; azs[nz++] = r; }
       }
-      // all (up to 8) contributions are loaded before the first is summed:
-      // predicated, fully unrolled, so the loads are in flight together
+      // all 8 contribution slots are loaded before the first is summed, so
+      // the loads are in flight together
       TY c8[8][VW];
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
         const int kz = k >> 2, ky = (k >> 1) & 1, kx = k & 1;
+        // no branch: a contribution the node does not have loads the node's
+        // first one (same cache line) and is then zeroed, so that all eight
+        // loads are issued before the first wait (behind a per-load branch
+        // the compiler waits for each load in turn)
+        const bool ok = kz < nz && ky < ny && kx < nx;
+        const int kz2 = ok ? kz : 0, ky2 = ok ? ky : 0, kx2 = ok ? kx : 0;
+        // the host checks cells n_e B < 2^32
+        const uint32_t e = (czs[kz2] * nc + cys[ky2]) * nc + cxs[kx2];
+        const uint32_t l = (azs[kz2] * n1 + ays[ky2]) * n1 + axs[kx2];
+        TY t[VW];
+        vload<TY, VW>(Yc + ((e * ne + l) * B + j0), t);
 #pragma unroll
-        for (int v = 0; v < VW; ++v) c8[k][v] = TY(0);
-        if (kz < nz && ky < ny && kx < nx) {
-          const uint64_t e = ((uint64_t)czs[kz] * nc + cys[ky]) * nc + cxs[kx];
-          const uint32_t l = (azs[kz] * n1 + ays[ky]) * n1 + axs[kx];
-          vload<TY, VW>(Yc + (e * ne + l) * B + j0, c8[k]);
-        }
+        for (int v = 0; v < VW; ++v) c8[k][v] = ok ? t[v] : TY(0);
       }
       double sum[VW];
 #pragma unroll
       for (int v = 0; v < VW; ++v) sum[v] = 0.0;
 #pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        const int kz = k >> 2, ky = (k >> 1) & 1, kx = k & 1;
-        if (kz < nz && ky < ny && kx < nx)
+      for (int k = 0; k < 8; ++k)
 #pragma unroll
-          for (int v = 0; v < VW; ++v) sum[v] += (double)c8[k][v];
-      }
+        for (int v = 0; v < VW; ++v) sum[v] += (double)c8[k][v];
       const uint32_t n = row * N1 + ix;
       const uint32_t i = n * B + j0;
       const double sn = xn ? xn[n] : 1.0;
@@ -1830,6 +1851,8 @@ int launch_cell_apply(const Geo& g, const CellOp<T>& op, const T* Sg, const TV* 
   }
   IDFT_HIP(hipGetLastError());
   check_u32(g.N * B);
+  check_u32(cell_scratch_elems(g, B));
+  const FastDiv divN1 = make_fastdiv((uint32_t)g.N1);
   unsigned gx, gy;
   uint32_t R;
   const int vw = gather_vw(B, (int)sizeof(TV));
@@ -1837,7 +1860,8 @@ int launch_cell_apply(const Geo& g, const CellOp<T>& op, const T* Sg, const TV* 
   dim3 grid(gx, gy);
 #define IDFT_GATHER(PP, VW)                                                                  \
   k_cell_gather<T, TV, PP, VW><<<grid, kThreads, 0, s>>>(                                    \
-      g, (uint32_t)B, R, Yc, S, ep.Y, ep.alpha, ep.beta, ep.X, ep.a, ep.ca, ep.Z, ep.b, ep.cb, \
+      g, divN1, (uint32_t)B, R, Yc, S, ep.Y, ep.alpha, ep.beta, ep.X, ep.a, ep.ca, ep.Z, ep.b,  \
+      ep.cb,                                                                                  \
       ep.W, ep.c, ep.cc, X, ep.xn, ep.xcol, ep.V, ep.dot_partial)
 #define IDFT_GATHER_P(PP)                                         \
   if (vw == 4) { if constexpr (sizeof(TV) == 4) IDFT_GATHER(PP, 4); } \
