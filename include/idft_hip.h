@@ -44,14 +44,13 @@ int idft_profile_apply(idft_ctx* ctx, int32_t ncols, int32_t precision, int32_t 
  *     (and 32 in FP32), wider tiles run the row pass.  The stored H_e is
  *     built on first use.
  *   IDFT_KERNEL_SUM_FACTOR: vectors_per_tile = columns per work-group, 2, 4
- *     or 8 (FP64) and also 16 (FP32); cells_per_wg = columns per work item,
- *     1 or 2 (FP64) and also 4 (FP32), at most vectors_per_tile. */
+ *     or 8; cells_per_wg must be 1 (one column per work item). */
 int idft_set_apply_shape(idft_ctx* ctx, int32_t precision, int32_t cells_per_wg,
                          int32_t vectors_per_tile, int32_t kernel, int32_t gather_rows);
 
 /* Autotuner of the cell operator: times the sum-factorised kernel at every
  * supported column count per work-group (2 only for blocks narrower than 8
- * columns) and per item, then every stored-H shape
+ * columns), then every stored-H shape
  * cells-per-work-group in {1,2,4,8} x vectors-per-tile in {16,32,64} (tiles
  * wider than the block collapse to one pass) x {row pass, tile pass} where
  * it applies, then node chunks per gather work-group in {default,1,2,4,8} with

@@ -448,7 +448,7 @@ class Context:
                         kernel=KERNEL_ROW_PASS):
         """Cell-product kernel and launch shape at one precision (HIP library
         only); for KERNEL_SUM_FACTOR vectors_per_tile is the number of columns
-        per work-group (2, 4, 8, and 16 in FP32)."""
+        per work-group (2, 4, 8) and cells_per_wg must be 1."""
         fn = self.lib.idft_set_apply_shape
         fn.restype = C.c_int
         fn.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32]

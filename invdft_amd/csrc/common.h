@@ -148,9 +148,9 @@ enum CellKernel {
 // Launch shape of the cell product (chosen by the autotuner).
 struct GemmShape {
   int cells_per_wg = 1;       // stored-H kernels: cells processed in sequence by one work-group;
-                              // sum-factorised kernel: columns per item (1, 2, 4)
+                              // sum-factorised kernel: columns per item (1)
   int vectors_per_tile = 80;  // stored-H kernels: columns per tile (rounded to 16; >= B: one
-                              // pass); sum-factorised kernel: columns per work-group (2, 4, 8, 16)
+                              // pass); sum-factorised kernel: columns per work-group (2, 4, 8)
   int gather_rows = 0;        // node chunks (256 / (B / VW) nodes each) per gather work-group
                               // (0: about kGatherBlocks groups)
   int kernel = CK_ROW_PASS;   // CellKernel; the tile pass applies to 16-column tiles (FP32 also 32)

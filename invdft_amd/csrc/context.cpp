@@ -1554,19 +1554,16 @@ int idft_autotune_apply(idft_ctx* c, int32_t ncols, int32_t prec, int32_t reps,
     };
     target.gather_rows = 0;
     target.kernel = CK_SUM_FACTOR;
-    for (int cg : {2, 4, 8, 16}) {
+    for (int cg : {2, 4, 8}) {
       if (cg > 2 && cg / 2 >= ncols) continue;  // mostly empty column groups
       // 2 columns per work-group gather each node row of the block once per
       // 2 columns.  Between the solvers' other vector kernels this measured
       // 1.3x slower than 4 columns at B = 12 (FP64 MINRES, kernel trace), a
       // cost that a timing loop of back-to-back applications does not show.
       if (cg == 2 && ncols >= 8) continue;
-      for (int vc : {1, 2, 4}) {
-        if (!sf_vc_supported(cg, vc, !f32)) continue;
-        target.vectors_per_tile = cg;
-        target.cells_per_wg = vc;
-        trial();
-      }
+      target.vectors_per_tile = cg;
+      target.cells_per_wg = 1;
+      trial();
     }
     const int tpMax = f32 ? 32 : 16;  // widest tile of the tile-pass kernel
     for (int vt : widths) {

@@ -511,7 +511,7 @@ template <typename T, int P, int I>
 __device__ constexpr T sfD() { return (T)SfC<P>::v.D[I]; }
 #define IDFT_IC(x) decltype(x)::value
 
-template <typename T, typename TV, int P, int CG, int VC>
+template <typename T, typename TV, int P, int CG, int VC, bool XN>
 __global__ __launch_bounds__(kThreads) void k_cell_sf(Geo g, T hq, const T* __restrict__ W,
                                                       const T* __restrict__ S,
                                                       const TV* __restrict__ X,
@@ -558,25 +558,36 @@ __global__ __launch_bounds__(kThreads) void k_cell_sf(Geo g, T hq, const T* __re
     const uint32_t b = ln % N1, c = ln / N1;
     const uint32_t n0 = ((cz * np + c) * NN + (cy * np + b)) * NN + cx * np;
     // the line's N1 nodes are consecutive: one base pointer per array, the
-    // nodes at immediate offsets (the X rows B apart)
+    // nodes at immediate offsets (the X rows B apart).  All loads of the line
+    // are issued before the first product (the node scale xn is a
+    // compile-time choice, so no branch separates them).
     const T* Sp = S + n0;
-    const double* xnp = xn ? xn + n0 : nullptr;
 #pragma unroll
     for (int u = 0; u < VC; ++u) {
       const int j = jv * VC + u;
       if (j < bw) {
         const TV* Xp = X + (size_t)n0 * ub + (uint32_t)(c0 + j);
         const double xc = xcol ? xcol[c0 + j] : 1.0;
+        T sv[N1];
+        TV xv[N1];
+        double nv[N1];
+#pragma unroll
+        for (int a = 0; a < N1; ++a) {
+          sv[a] = Sp[a];
+          xv[a] = Xp[(size_t)a * ub];
+          if constexpr (XN) nv[a] = xn[n0 + a];
+          else nv[a] = 1.0;
+        }
 #pragma unroll
         for (int a = 0; a < N1; ++a) {
           if constexpr (std::is_same<T, TV>::value) {
-            T x = Sp[a] * Xp[(size_t)a * ub];
-            if (xnp) x *= (T)xnp[a];
+            T x = sv[a] * xv[a];
+            if constexpr (XN) x *= (T)nv[a];
             if (xcol) x *= (T)xc;
             v[a].v[u] = x;
           } else {  // wider input vectors: scale in FP64, round once
-            double x = (double)Sp[a] * (double)Xp[(size_t)a * ub];
-            if (xnp) x *= xnp[a];
+            double x = (double)sv[a] * (double)xv[a];
+            if constexpr (XN) x *= nv[a];
             if (xcol) x *= xc;
             v[a].v[u] = (T)x;
           }
@@ -1693,41 +1704,35 @@ void launch_build_hamiltonian(const Geo& g, const DevTables& t, double hh, const
 }
 
 // Columns per work-group (CG) and per item (VC) of the sum-factorised
-// kernel: its LDS holds about 1644 CG values at p = 5, so FP64 stops at 8
-// columns; VC in {1, 2, 4} divides CG, FP64 up to 2.
-bool sf_cols_supported(int cols, bool fp64) {
-  return cols == 2 || cols == 4 || cols == 8 || (!fp64 && cols == 16);
-}
+// kernel built into the library: CG in {2, 4, 8} (its LDS holds about 1644 CG
+// values at p = 5, so FP64 stops at 8), one column per item.  The kernel is
+// written for VC columns per item; wider items and 16-column groups measured
+// slower on every mesh tried and are not instantiated.
+bool sf_cols_supported(int cols, bool) { return cols == 2 || cols == 4 || cols == 8; }
 bool sf_vc_supported(int cols, int vc, bool fp64) {
-  return sf_cols_supported(cols, fp64) && (vc == 1 || vc == 2 || (!fp64 && vc == 4)) &&
-         vc <= cols;
+  return sf_cols_supported(cols, fp64) && vc == 1;
 }
 
 template <typename T, typename TV, int P, int CG>
 static void cell_sf_cg(const Geo& g, const CellOp<T>& op, const T* S, const TV* X,
-                       const double* xn, const double* xcol, T* Yc, int B, int vc,
-                       hipStream_t s) {
+                       const double* xn, const double* xcol, T* Yc, int B, hipStream_t s) {
   check_u32(g.N * B);
   check_u32(cell_scratch_elems(g, B));
   const int64_t npair = g.ncell * ((B + CG - 1) / CG);
   check_u32(8 * ((npair + 7) / 8));
   dim3 grid((unsigned)(8 * ((npair + 7) / 8))), block(kThreads);
   const T hq = (T)op.kin_scale;
-  if (vc == 1) {
-    k_cell_sf<T, TV, P, CG, 1><<<grid, block, 0, s>>>(g, hq, op.W, S, X, xn, xcol, Yc, B);
-    return;
-  }
-  if (vc == 2) {
-    k_cell_sf<T, TV, P, CG, 2><<<grid, block, 0, s>>>(g, hq, op.W, S, X, xn, xcol, Yc, B);
-    return;
-  }
-  if constexpr (std::is_same<T, float>::value && CG >= 4) {
-    if (vc == 4) {
-      k_cell_sf<T, TV, P, CG, 4><<<grid, block, 0, s>>>(g, hq, op.W, S, X, xn, xcol, Yc, B);
+  // node-scaled inputs (MINRES) come as FP64 vectors; FP32 vectors never
+  // carry a node scale
+  if constexpr (std::is_same<TV, double>::value) {
+    if (xn) {
+      k_cell_sf<T, TV, P, CG, 1, true><<<grid, block, 0, s>>>(g, hq, op.W, S, X, xn, xcol, Yc, B);
       return;
     }
+  } else {
+    if (xn) throw Error(IDFT_EINVAL, "node-scaled input needs FP64 vectors");
   }
-  throw Error(IDFT_EINVAL, "columns per item not supported by the sum-factorised kernel");
+  k_cell_sf<T, TV, P, CG, 1, false><<<grid, block, 0, s>>>(g, hq, op.W, S, X, nullptr, xcol, Yc, B);
 }
 
 template <typename T, typename TV, int P>
@@ -1738,15 +1743,9 @@ static void cell_sf_p(const Geo& g, const CellOp<T>& op, const T* S, const TV* X
   if (!sf_vc_supported(cg, vc, std::is_same<T, double>::value))
     throw Error(IDFT_EINVAL, "shape not supported by the sum-factorised kernel");
   switch (cg) {
-    case 2: cell_sf_cg<T, TV, P, 2>(g, op, S, X, xn, xcol, Yc, B, vc, s); break;
-    case 4: cell_sf_cg<T, TV, P, 4>(g, op, S, X, xn, xcol, Yc, B, vc, s); break;
-    case 8: cell_sf_cg<T, TV, P, 8>(g, op, S, X, xn, xcol, Yc, B, vc, s); break;
-    case 16:
-      if constexpr (std::is_same<T, float>::value) {
-        cell_sf_cg<T, TV, P, 16>(g, op, S, X, xn, xcol, Yc, B, vc, s);
-        break;
-      }
-      [[fallthrough]];
+    case 2: cell_sf_cg<T, TV, P, 2>(g, op, S, X, xn, xcol, Yc, B, s); break;
+    case 4: cell_sf_cg<T, TV, P, 4>(g, op, S, X, xn, xcol, Yc, B, s); break;
+    case 8: cell_sf_cg<T, TV, P, 8>(g, op, S, X, xn, xcol, Yc, B, s); break;
     default: throw Error(IDFT_EINVAL, "columns per work-group not supported by the sum-factorised kernel");
   }
 }
