@@ -1616,17 +1616,37 @@ __global__ __launch_bounds__(kThreads) void k_minres_givens(int B, int nblk,
 
 // w_new = cwv D r + cww1 w1 + cww w  (written over w1);  x += cx w_new,
 // where cwv D r = v / gamma, v = cv D r the step's Lanczos vector.
-template <typename T>
+template <typename T, int VW>
 __global__ void k_minres_update(uint32_t NB, uint32_t B, const T* __restrict__ r,
                                 const double* __restrict__ D, T* __restrict__ w1,
                                 const T* __restrict__ w, T* __restrict__ x,
                                 const double* __restrict__ st) {
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < NB; i += gridDim.x * blockDim.x) {
-    const uint32_t n = i / B, j = i - n * B;
-    const double wn = st[MS_CWV * B + j] * (D[n] * (double)r[i]) +
-                      st[MS_CWW1 * B + j] * (double)w1[i] + st[MS_CWW * B + j] * (double)w[i];
-    w1[i] = (T)wn;
-    x[i] = (T)((double)x[i] + st[MS_CX * B + j] * wn);
+  // VW consecutive columns of one node per thread (B % VW == 0), so every
+  // access is one VW-wide load or store
+  const uint32_t NG = NB / VW;
+  for (uint32_t gi = blockIdx.x * blockDim.x + threadIdx.x; gi < NG; gi += gridDim.x * blockDim.x) {
+    const uint32_t i = gi * VW, n = i / B, j = i - n * B;
+    T rv[VW], w1v[VW], wv[VW], xv[VW];
+    double cwv[VW], cw1[VW], cww[VW], cx[VW];
+    vload<T, VW>(r + i, rv);
+    vload<T, VW>(w1 + i, w1v);
+    vload<T, VW>(w + i, wv);
+    vload<T, VW>(x + i, xv);
+    vload<double, VW>(st + MS_CWV * B + j, cwv);
+    vload<double, VW>(st + MS_CWW1 * B + j, cw1);
+    vload<double, VW>(st + MS_CWW * B + j, cww);
+    vload<double, VW>(st + MS_CX * B + j, cx);
+    const double dn = D[n];
+    T wo[VW], xo[VW];
+#pragma unroll
+    for (int v = 0; v < VW; ++v) {
+      const double wn = cwv[v] * (dn * (double)rv[v]) + cw1[v] * (double)w1v[v] +
+                        cww[v] * (double)wv[v];
+      wo[v] = (T)wn;
+      xo[v] = (T)((double)xv[v] + cx[v] * wn);
+    }
+    vstore<T, VW>(w1 + i, wo);
+    vstore<T, VW>(x + i, xo);
   }
 }
 
@@ -2119,8 +2139,12 @@ template <typename T>
 void launch_minres_update(int64_t N, int B, const T* r, const double* D, T* w1, const T* w, T* x,
                           const double* st, hipStream_t s) {
   check_u32(N * B);
-  k_minres_update<T><<<grid_for(N * B), kThreads, 0, s>>>((uint32_t)(N * B), (uint32_t)B, r, D,
-                                                          w1, w, x, st);
+  if (B % 2 == 0)
+    k_minres_update<T, 2><<<grid_for(N * B / 2), kThreads, 0, s>>>((uint32_t)(N * B), (uint32_t)B,
+                                                                  r, D, w1, w, x, st);
+  else
+    k_minres_update<T, 1><<<grid_for(N * B), kThreads, 0, s>>>((uint32_t)(N * B), (uint32_t)B, r,
+                                                               D, w1, w, x, st);
   IDFT_HIP(hipGetLastError());
 }
 template void launch_minres_update<float>(int64_t, int, const float*, const double*, float*,
