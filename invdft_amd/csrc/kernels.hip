@@ -172,24 +172,60 @@ __device__ inline void load_cell_input(const Geo& g, int64_t cx, int64_t cy, int
                                        const double* __restrict__ xn,
                                        const double* __restrict__ xcol, int B, int c0, int bw,
                                        T* xs) {
-  constexpr int NE = ne_of(P), NEP = nep_of(P);
-  for (int i = threadIdx.x; i < NEP * BT; i += NTH) {
-    const int l = i / BT, j = i % BT;
-    T v = T(0);
-    if (l < NE && j < bw) {
-      const int64_t n = cell_node(g, cx, cy, cz, l);
-      if constexpr (std::is_same<T, TV>::value) {
-        v = S[n] * X[n * B + c0 + j];
-        if (xn) v *= (T)xn[n];
-        if (xcol) v *= (T)xcol[c0 + j];
-      } else {  // wider input vectors: scale in FP64, round once
-        double w = (double)S[n] * (double)X[n * B + c0 + j];
-        if (xn) w *= xn[n];
-        if (xcol) w *= xcol[c0 + j];
-        v = (T)w;
+  constexpr int NE = ne_of(P), NEP = nep_of(P), TOT = NEP * BT;
+  // U entries per thread per round, all loads of a round issued before the
+  // first product: the optional scales are loaded inside uniform branches
+  // that hold nothing else, so no product is pulled in after its own load
+  constexpr int U = 4;
+  for (int i0 = threadIdx.x; i0 < TOT; i0 += NTH * U) {
+    int64_t n[U];
+    int col[U];
+    bool ok[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = i0 + u * NTH;
+      const int l = i / BT, j = i % BT;
+      ok[u] = i < TOT && l < NE && j < bw;
+      n[u] = ok[u] ? cell_node(g, cx, cy, cz, l) : 0;
+      col[u] = ok[u] ? c0 + j : 0;
+    }
+    T sv[U];
+    TV xv[U];
+    double nv[U], cv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      sv[u] = S[n[u]];
+      xv[u] = X[n[u] * B + col[u]];
+    }
+    if (xn) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) nv[u] = xn[n[u]];
+    } else {
+#pragma unroll
+      for (int u = 0; u < U; ++u) nv[u] = 1.0;
+    }
+    if (xcol) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) cv[u] = xcol[col[u]];
+    } else {
+#pragma unroll
+      for (int u = 0; u < U; ++u) cv[u] = 1.0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = i0 + u * NTH;
+      if (i < TOT) {
+        T v = T(0);
+        if (ok[u]) {
+          if constexpr (std::is_same<T, TV>::value) {
+            v = sv[u] * xv[u] * (T)nv[u] * (T)cv[u];
+          } else {  // wider input vectors: scale in FP64, round once
+            v = (T)((double)sv[u] * (double)xv[u] * nv[u] * cv[u]);
+          }
+        }
+        xs[(i / BT) * LDB + i % BT] = v;
       }
     }
-    xs[l * LDB + j] = v;
   }
 }
 
