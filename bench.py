@@ -50,7 +50,7 @@ def sf_flops_per_cell_column(p):
 # with the commands are profiles/cell_sf_inverse_24c_p5_pmc.md (shape 1 4 4 2)
 # and profiles/cell_gemm_inverse_24c_p5_pmc.md (2 16 4 0 and 1 16 0 1).
 MEASURED_TRAFFIC = {
-    ("inverse_24c_p5", "k_cell_sf<float,float,5,4,1>"): 276.3e6,
+    ("inverse_24c_p5", "k_cell_sf<float,float,5,4,1,false>"): 276.3e6,
     ("inverse_24c_p5", "k_cell_gemm<float,float,5,1>"): 2613e6,
     ("inverse_24c_p5", "k_cell_symm<float,float,5,1>"): 1691e6,
 }
@@ -347,8 +347,9 @@ def main():
     tuned32 = mixed["tuned"]["fp32"]
     kern = tuned32["kernel"]
     if kern == abi.KERNEL_SUM_FACTOR:
+        # the plain application's instantiation (no node scale on the input)
         kname = (f"k_cell_sf<float,float,{p},{tuned32['vectors_per_tile']},"
-                 f"{tuned32['cells_per_wg']}>")
+                 f"{tuned32['cells_per_wg']},false>")
         fl = sf_flops()
         tf = fl / (ms_gemm * 1e-3) / 1e12
         roof = {"bound": "valu", "achieved": round(tf, 2), "peak": FP32_PEAK_TFLOPS,
