@@ -1440,10 +1440,26 @@ __global__ __launch_bounds__(kThreads) void k_gram(int64_t N, int B, const doubl
   for (int64_t n0 = (int64_t)blockIdx.x * kGramChunk; n0 < N; n0 += (int64_t)gridDim.x * kGramChunk) {
     const int cnt = (int)((N - n0) < kGramChunk ? (N - n0) : kGramChunk);
     __syncthreads();
-    for (int t = tid; t < kGramChunk * B; t += kThreads) {
-      const int r = t / B;
-      xs[t] = r < cnt ? X[n0 * B + t] : 0.0;
-      ys[t] = r < cnt ? Y[n0 * B + t] : 0.0;
+    // four entries per thread per round, their loads issued together
+    for (int t0 = tid; t0 < kGramChunk * B; t0 += 4 * kThreads) {
+      double xv[4], yv[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int t = t0 + u * kThreads;
+        const bool ok = t < kGramChunk * B && t / B < cnt;
+        const int64_t g = ok ? n0 * B + t : 0;
+        xv[u] = X[g];
+        yv[u] = Y[g];
+        if (!ok) xv[u] = yv[u] = 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int t = t0 + u * kThreads;
+        if (t < kGramChunk * B) {
+          xs[t] = xv[u];
+          ys[t] = yv[u];
+        }
+      }
     }
     __syncthreads();
     if (ij < BB)
@@ -1464,6 +1480,7 @@ __global__ void k_right_mult(uint32_t NB, uint32_t B, const double* __restrict__
     const uint32_t j = t - n * B;
     const double* xr = X + (size_t)n * B;
     double s = 0.0;
+#pragma unroll 4
     for (uint32_t i = 0; i < B; ++i) s += xr[i] * qs[i * B + j];
     if (ns) s *= ns[n];
     Y[t] = beta != 0.0 ? beta * Y[t] + s : s;
