@@ -654,10 +654,11 @@ __global__ __launch_bounds__(kThreads) void k_cell_sf(Geo g, T hq, const T* __re
     });
   }
   __syncthreads();
-  // stage 2: y lines -- potential (c, qx), then kinetic (c, a)
-  for (int it = tid; it < (N1 * Q + N2) * NV; it += kThreads) {
+  // stage 2: y lines -- potential (c, qx), then kinetic (c, a), as two
+  // loops so that each holds only its own tables' constants
+  for (int it = tid; it < N1 * Q * NV; it += kThreads) {
     const int jv = it % NV, ln = it / NV;
-    if (ln < N1 * Q) {
+    {
       const int qx = ln % Q, c = ln / Q;
       pk v[N1];
 #pragma unroll
@@ -670,8 +671,11 @@ __global__ __launch_bounds__(kThreads) void k_cell_sf(Geo g, T hq, const T* __re
         });
         t2[((c * Q + IDFT_IC(qy)) * Q + qx) * NV + jv] = s;
       });
-    } else {
-      const int l2 = ln - N1 * Q;
+    }
+  }
+  for (int it = tid; it < N2 * NV; it += kThreads) {
+    const int jv = it % NV, l2 = it / NV;
+    {
       const int a = l2 % N1, c = l2 / N1;
       pk va[N1], vb[N1];
 #pragma unroll
@@ -696,10 +700,10 @@ __global__ __launch_bounds__(kThreads) void k_cell_sf(Geo g, T hq, const T* __re
   }
   __syncthreads();
   // stage 3: z lines -- potential (qy, qx) through the quadrature weights,
-  // then kinetic (b, a)
-  for (int it = tid; it < (Q2 + N2) * NV; it += kThreads) {
+  // then kinetic (b, a), again as two loops
+  for (int it = tid; it < Q2 * NV; it += kThreads) {
     const int jv = it % NV, ln = it / NV;
-    if (ln < Q2) {
+    {
       pk v[N1];
 #pragma unroll
       for (int c = 0; c < N1; ++c) v[c] = t2[(c * Q2 + ln) * NV + jv];
@@ -722,8 +726,11 @@ __global__ __launch_bounds__(kThreads) void k_cell_sf(Geo g, T hq, const T* __re
         });
         s2[(IDFT_IC(c) * Q2 + ln) * NV + jv] = s;
       });
-    } else {
-      const int l2 = ln - Q2;  // b * N1 + a
+    }
+  }
+  for (int it = tid; it < N2 * NV; it += kThreads) {
+    const int jv = it % NV, l2 = it / NV;  // l2 = b * N1 + a
+    {
       const int a = l2 % N1, b = l2 / N1;
       pk v1[N1], v2[N1];
 #pragma unroll
