@@ -34,16 +34,31 @@ FP32_PEAK_TFLOPS = 157.3  # MI355X published FP32 vector / dense matrix peak
 
 
 def sf_flops_per_cell_column(p):
-    """Flops of the sum-factorised cell product per cell and column: the 1D
-    contractions of its five stages (2 per multiply-add), the quadrature
-    weight multiply and the kinetic scale."""
+    """Flops of the sum-factorised cell product per cell and column.  Every 1D
+    contraction y = A x (A of R x C, centro-symmetric) runs in even-odd form:
+    C/2 sums and differences of the input, ceil(R/2) even rows of
+    ceil(C/2) multiply-adds, floor(R/2) odd rows of floor(C/2), and one sum
+    and one difference per output pair.  Two tables into one output (the
+    kinetic D + M lines) share the output pairs.  A multiply-add counts 2."""
     n1, q = p + 1, p + 3
-    fma = (n1 ** 3 * (q + 2 * n1)            # x lines: Lq, M, D
-           + n1 ** 2 * q ** 2 + 3 * n1 ** 4  # y lines: Lq; M, D + M
-           + 2 * q ** 3 * n1 + 2 * n1 ** 4   # z lines: Lq and Lq^T; D + M
-           + n1 ** 2 * q ** 2                # y lines back
-           + n1 ** 3 * q)                    # x lines back
-    return 2 * fma + q ** 3 + n1 ** 3
+
+    def split(c):
+        return 2 * (c // 2)
+
+    def contract(r, c, tables=1):
+        fma = tables * (((r + 1) // 2) * ((c + 1) // 2) + (r // 2) * (c // 2))
+        return 2 * fma + 2 * (r // 2)
+
+    x_lines = n1 ** 2 * (split(n1) + contract(q, n1) + 2 * contract(n1, n1))
+    y_lines = (n1 * q * (split(n1) + contract(q, n1))
+               + n1 ** 2 * (2 * split(n1) + contract(n1, n1) + contract(n1, n1, 2)))
+    z_lines = (q ** 2 * (split(n1) + contract(q, n1) + q + split(q) + contract(n1, q))
+               + n1 ** 2 * (2 * split(n1) + contract(n1, n1, 2)))
+    y_back = n1 * q * (split(q) + contract(n1, q))
+    x_back = n1 ** 2 * (split(q) + contract(n1, q) + 2 * n1)  # + (h/4) kinetic, fused
+    return x_lines + y_lines + z_lines + y_back + x_back
+
+
 # HBM bytes per launch of the dominant kernel from two rocprofv3 --pmc runs
 # (FETCH_SIZE, WRITE_SIZE) of `bench.py --kernels-only --shape ...`, FETCH_SIZE
 # scaled by the copy calibration (x1.926), WRITE_SIZE as read; the summaries

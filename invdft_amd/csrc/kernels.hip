@@ -501,11 +501,12 @@ __global__ __launch_bounds__(kSymmThreads) void k_cell_symm(Geo g, const T* __re
 // ----------------------------------------------- sum-factorised cell product
 // Phase 1 without stored cell matrices.  On the uniform mesh
 //   H_e = (h/4)(D(x)M(x)M + M(x)D(x)M + M(x)M(x)D) + L^T diag(w_e) L,  L = Lq(x)Lq(x)Lq,
-// so H_e X_e is a chain of 1D contractions along x, y and z: 23 k FMA per
-// column at p = 5 (the dense product is 47 k), and per cell only the (p+3)^3
-// weights w_e are read from HBM instead of n_e(n_e+1)/2 entries of H_e.  The
-// 1D tables are compile-time constants (SfC<P>), so after unrolling every
-// coefficient is an immediate operand; h/4 scales the kinetic sum once.
+// so H_e X_e is a chain of 1D contractions along x, y and z, each in the
+// even-odd form below: 11.9 k FMA and 6.6 k adds per column at p = 5 (23 k FMA
+// without the even-odd split, 47 k for the dense product), and per cell only
+// the (p+3)^3 weights w_e are read from HBM instead of n_e(n_e+1)/2 entries of
+// H_e.  The 1D tables are compile-time constants (SfC<P>), so after unrolling
+// every coefficient is an immediate operand; h/4 scales the kinetic sum once.
 //
 // One work-group handles one cell and CG columns; an item is one line of the
 // cell's tensor along one axis for VC consecutive columns (VC-wide LDS
@@ -537,14 +538,93 @@ __device__ inline void pack_fma(Pack<T, VC>& acc, T c, const Pack<T, VC>& x) {
   for (int u = 0; u < VC; ++u) acc.v[u] += c * x.v[u];
 }
 
-// Compile-time coefficients of the 1D tables (constant expressions, so each
-// is an immediate operand rather than a load from a constant global).
-template <typename T, int P, int I>
-__device__ constexpr T sfL() { return (T)SfC<P>::v.L[I]; }
-template <typename T, int P, int I>
-__device__ constexpr T sfM() { return (T)SfC<P>::v.M[I]; }
-template <typename T, int P, int I>
-__device__ constexpr T sfD() { return (T)SfC<P>::v.D[I]; }
+// The 1D tables as operators A (R x C): K = 0 Lq (Gauss points from nodes),
+// 1 Lq^T, 2 M, 3 D.  The GLL nodes and the Gauss points lie symmetrically
+// about the cell centre, so each A is centro-symmetric,
+// A[R-1-i][C-1-j] = A[i][j] (to rounding of the tables).
+template <int P, int K>
+struct SfTab {
+  static constexpr int N1 = P + 1, Q = P + 3;
+  static constexpr int R = K == 0 ? Q : N1;
+  static constexpr int C = K == 1 ? Q : N1;
+  __host__ __device__ static constexpr double at(int i, int j) {
+    return K == 0   ? SfC<P>::v.L[i * N1 + j]
+           : K == 1 ? SfC<P>::v.L[j * N1 + i]
+           : K == 2 ? SfC<P>::v.M[i * N1 + j]
+                    : SfC<P>::v.D[i * N1 + j];
+  }
+};
+// Even-odd form of y = A x (Kopriva's even-odd decomposition).  With
+// x+_j = x_j + x_{C-1-j} and x-_j = x_j - x_{C-1-j} for j < C/2 (the middle
+// entry of an odd C kept alone as x+_{C/2}), row i < R/2 takes
+//   e_i = sum_j (A_ij + A_i,C-1-j)/2 x+_j [+ A_i,mid x_mid]
+//   o_i = sum_j (A_ij - A_i,C-1-j)/2 x-_j
+// and y_i = e_i + o_i, y_{R-1-i} = e_i - o_i; the middle row of an odd R is
+// e alone.  R C / 2 multiply-adds instead of R C.  The coefficients are
+// constant expressions, so each is an immediate operand after unrolling.
+template <typename T, class A, int I, int J>
+__device__ constexpr T eo_even() {
+  constexpr double c = 0.5 * (A::at(I, J) + A::at(I, A::C - 1 - J));
+  return (T)c;
+}
+template <typename T, class A, int I, int J>
+__device__ constexpr T eo_odd() {
+  constexpr double c = 0.5 * (A::at(I, J) - A::at(I, A::C - 1 - J));
+  return (T)c;
+}
+template <typename T, class A, int I>
+__device__ constexpr T eo_mid() {
+  constexpr double c = A::at(I, A::C / 2);
+  return (T)c;
+}
+template <int C, typename T, int VC>
+__device__ inline void eo_split(const Pack<T, VC> (&x)[C], Pack<T, VC> (&xp)[(C + 1) / 2],
+                                Pack<T, VC> (&xm)[C / 2]) {
+#pragma unroll
+  for (int j = 0; j < C / 2; ++j)
+#pragma unroll
+    for (int u = 0; u < VC; ++u) {
+      xp[j].v[u] = x[j].v[u] + x[C - 1 - j].v[u];
+      xm[j].v[u] = x[j].v[u] - x[C - 1 - j].v[u];
+    }
+  if constexpr (C % 2 == 1) xp[C / 2] = x[C / 2];
+}
+// Adds row I's even and odd sums of A applied to the split input (xp, xm).
+template <typename T, class A, int I, int VC>
+__device__ inline void eo_row(const Pack<T, VC> (&xp)[(A::C + 1) / 2],
+                              const Pack<T, VC> (&xm)[A::C / 2], Pack<T, VC>& e,
+                              Pack<T, VC>& o) {
+  static_for<0, A::C / 2>([&](auto j) {
+    constexpr int J = decltype(j)::value;
+    pack_fma(e, eo_even<T, A, I, J>(), xp[J]);
+    if constexpr (2 * I + 1 != A::R) pack_fma(o, eo_odd<T, A, I, J>(), xm[J]);
+  });
+  if constexpr (A::C % 2 == 1) pack_fma(e, eo_mid<T, A, I>(), xp[A::C / 2]);
+}
+// The R outputs of one or more even-odd contractions: rows(i, e, o) adds the
+// sums of row i, put(i, y) receives output i (i an integral constant).
+template <int R, typename T, int VC, typename Rows, typename Put>
+__device__ inline void eo_out(Rows&& rows, Put&& put) {
+  static_for<0, (R + 1) / 2>([&](auto i) {
+    constexpr int I = decltype(i)::value;
+    Pack<T, VC> e, o;
+    pack_zero(e);
+    pack_zero(o);
+    rows(i, e, o);
+    if constexpr (2 * I + 1 == R) {
+      put(i, e);
+    } else {
+      Pack<T, VC> y0, y1;
+#pragma unroll
+      for (int u = 0; u < VC; ++u) {
+        y0.v[u] = e.v[u] + o.v[u];
+        y1.v[u] = e.v[u] - o.v[u];
+      }
+      put(i, y0);
+      put(ic<R - 1 - I>{}, y1);
+    }
+  });
+}
 #define IDFT_IC(x) decltype(x)::value
 
 template <typename T, typename TV, int P, int CG, int VC, bool XN>
@@ -559,6 +639,10 @@ __global__ __launch_bounds__(kThreads) void k_cell_sf(Geo g, T hq, const T* __re
   constexpr int SZ1 = cmax(N2 * QP + 2 * N2 * N1P, N1 * Q2 + N2 * N1P);
   constexpr int SZ2 = cmax(N1 * Q2 + 2 * NE, N2 * QP);
   typedef Pack<T, VC> pk;
+  typedef SfTab<P, 0> TL;
+  typedef SfTab<P, 1> TLt;
+  typedef SfTab<P, 2> TM;
+  typedef SfTab<P, 3> TD;
   __shared__ pk r1[SZ1 * NV];
   __shared__ pk r2[SZ2 * NV];
   // (cell, column group) pairs dealt to the 8 XCDs in contiguous runs, the
@@ -633,161 +717,118 @@ __global__ __launch_bounds__(kThreads) void k_cell_sf(Geo g, T hq, const T* __re
         for (int a = 0; a < N1; ++a) v[a].v[u] = T(0);
       }
     }
-    static_for<0, Q>([&](auto q) {
-      pk s;
-      pack_zero(s);
-      static_for<0, N1>([&](auto a) {
-        pack_fma(s, sfL<T, P, IDFT_IC(q) * N1 + IDFT_IC(a)>(), v[IDFT_IC(a)]);
-      });
-      t1[(ln * QP + IDFT_IC(q)) * NV + jv] = s;
-    });
-    static_for<0, N1>([&](auto a2) {
-      pk sm, sd;
-      pack_zero(sm);
-      pack_zero(sd);
-      static_for<0, N1>([&](auto a) {
-        pack_fma(sm, sfM<T, P, IDFT_IC(a2) * N1 + IDFT_IC(a)>(), v[IDFT_IC(a)]);
-        pack_fma(sd, sfD<T, P, IDFT_IC(a2) * N1 + IDFT_IC(a)>(), v[IDFT_IC(a)]);
-      });
-      ka[(ln * N1P + IDFT_IC(a2)) * NV + jv] = sm;
-      kb[(ln * N1P + IDFT_IC(a2)) * NV + jv] = sd;
-    });
+    pk vp[(N1 + 1) / 2], vm[N1 / 2];
+    eo_split(v, vp, vm);
+    eo_out<Q, T, VC>([&](auto i, pk& s, pk& d) { eo_row<T, TL, IDFT_IC(i)>(vp, vm, s, d); },
+                     [&](auto q, const pk& r) { t1[(ln * QP + IDFT_IC(q)) * NV + jv] = r; });
+    eo_out<N1, T, VC>([&](auto i, pk& s, pk& d) { eo_row<T, TM, IDFT_IC(i)>(vp, vm, s, d); },
+                      [&](auto a2, const pk& r) { ka[(ln * N1P + IDFT_IC(a2)) * NV + jv] = r; });
+    eo_out<N1, T, VC>([&](auto i, pk& s, pk& d) { eo_row<T, TD, IDFT_IC(i)>(vp, vm, s, d); },
+                      [&](auto a2, const pk& r) { kb[(ln * N1P + IDFT_IC(a2)) * NV + jv] = r; });
   }
   __syncthreads();
   // stage 2: y lines -- potential (c, qx), then kinetic (c, a), as two
   // loops so that each holds only its own tables' constants
   for (int it = tid; it < N1 * Q * NV; it += kThreads) {
     const int jv = it % NV, ln = it / NV;
-    {
-      const int qx = ln % Q, c = ln / Q;
-      pk v[N1];
+    const int qx = ln % Q, c = ln / Q;
+    pk v[N1], vp[(N1 + 1) / 2], vm[N1 / 2];
 #pragma unroll
-      for (int b = 0; b < N1; ++b) v[b] = t1[((c * N1 + b) * QP + qx) * NV + jv];
-      static_for<0, Q>([&](auto qy) {
-        pk s;
-        pack_zero(s);
-        static_for<0, N1>([&](auto b) {
-          pack_fma(s, sfL<T, P, IDFT_IC(qy) * N1 + IDFT_IC(b)>(), v[IDFT_IC(b)]);
-        });
-        t2[((c * Q + IDFT_IC(qy)) * Q + qx) * NV + jv] = s;
-      });
-    }
+    for (int b = 0; b < N1; ++b) v[b] = t1[((c * N1 + b) * QP + qx) * NV + jv];
+    eo_split(v, vp, vm);
+    eo_out<Q, T, VC>(
+        [&](auto i, pk& s, pk& d) { eo_row<T, TL, IDFT_IC(i)>(vp, vm, s, d); },
+        [&](auto qy, const pk& r) { t2[((c * Q + IDFT_IC(qy)) * Q + qx) * NV + jv] = r; });
   }
   for (int it = tid; it < N2 * NV; it += kThreads) {
     const int jv = it % NV, l2 = it / NV;
-    {
-      const int a = l2 % N1, c = l2 / N1;
-      pk va[N1], vb[N1];
+    const int a = l2 % N1, c = l2 / N1;
+    pk va[N1], vb[N1], ap[(N1 + 1) / 2], am[N1 / 2], bp[(N1 + 1) / 2], bm[N1 / 2];
 #pragma unroll
-      for (int b = 0; b < N1; ++b) {
-        va[b] = ka[((c * N1 + b) * N1P + a) * NV + jv];
-        vb[b] = kb[((c * N1 + b) * N1P + a) * NV + jv];
-      }
-      static_for<0, N1>([&](auto b2) {
-        pk s1v, s2v;
-        pack_zero(s1v);
-        pack_zero(s2v);
-        static_for<0, N1>([&](auto b) {
-          constexpr int I = IDFT_IC(b2) * N1 + IDFT_IC(b);
-          pack_fma(s1v, sfM<T, P, I>(), va[IDFT_IC(b)]);
-          pack_fma(s2v, sfD<T, P, I>(), va[IDFT_IC(b)]);
-          pack_fma(s2v, sfM<T, P, I>(), vb[IDFT_IC(b)]);
-        });
-        k1[((c * N1 + IDFT_IC(b2)) * N1 + a) * NV + jv] = s1v;
-        k2[((c * N1 + IDFT_IC(b2)) * N1 + a) * NV + jv] = s2v;
-      });
+    for (int b = 0; b < N1; ++b) {
+      va[b] = ka[((c * N1 + b) * N1P + a) * NV + jv];
+      vb[b] = kb[((c * N1 + b) * N1P + a) * NV + jv];
     }
+    eo_split(va, ap, am);
+    eo_split(vb, bp, bm);
+    eo_out<N1, T, VC>(
+        [&](auto i, pk& s, pk& d) { eo_row<T, TM, IDFT_IC(i)>(ap, am, s, d); },
+        [&](auto b2, const pk& r) { k1[((c * N1 + IDFT_IC(b2)) * N1 + a) * NV + jv] = r; });
+    eo_out<N1, T, VC>(
+        [&](auto i, pk& s, pk& d) {
+          eo_row<T, TD, IDFT_IC(i)>(ap, am, s, d);
+          eo_row<T, TM, IDFT_IC(i)>(bp, bm, s, d);
+        },
+        [&](auto b2, const pk& r) { k2[((c * N1 + IDFT_IC(b2)) * N1 + a) * NV + jv] = r; });
   }
   __syncthreads();
   // stage 3: z lines -- potential (qy, qx) through the quadrature weights,
   // then kinetic (b, a), again as two loops
   for (int it = tid; it < Q2 * NV; it += kThreads) {
     const int jv = it % NV, ln = it / NV;
-    {
-      pk v[N1];
+    pk v[N1], vp[(N1 + 1) / 2], vm[N1 / 2];
 #pragma unroll
-      for (int c = 0; c < N1; ++c) v[c] = t2[(c * Q2 + ln) * NV + jv];
-      pk u[Q];
-      static_for<0, Q>([&](auto qz) {
-        pk s;
-        pack_zero(s);
-        static_for<0, N1>([&](auto c) {
-          pack_fma(s, sfL<T, P, IDFT_IC(qz) * N1 + IDFT_IC(c)>(), v[IDFT_IC(c)]);
-        });
-        const T wq = we[IDFT_IC(qz) * Q2 + ln];
+    for (int c = 0; c < N1; ++c) v[c] = t2[(c * Q2 + ln) * NV + jv];
+    eo_split(v, vp, vm);
+    pk u[Q], up[(Q + 1) / 2], um[Q / 2];
+    eo_out<Q, T, VC>([&](auto i, pk& s, pk& d) { eo_row<T, TL, IDFT_IC(i)>(vp, vm, s, d); },
+                     [&](auto qz, const pk& r) {
+                       const T wq = we[IDFT_IC(qz) * Q2 + ln];
 #pragma unroll
-        for (int uu = 0; uu < VC; ++uu) u[IDFT_IC(qz)].v[uu] = s.v[uu] * wq;
-      });
-      static_for<0, N1>([&](auto c) {
-        pk s;
-        pack_zero(s);
-        static_for<0, Q>([&](auto qz) {
-          pack_fma(s, sfL<T, P, IDFT_IC(qz) * N1 + IDFT_IC(c)>(), u[IDFT_IC(qz)]);
-        });
-        s2[(IDFT_IC(c) * Q2 + ln) * NV + jv] = s;
-      });
-    }
+                       for (int uu = 0; uu < VC; ++uu) u[IDFT_IC(qz)].v[uu] = r.v[uu] * wq;
+                     });
+    eo_split(u, up, um);
+    eo_out<N1, T, VC>([&](auto i, pk& s, pk& d) { eo_row<T, TLt, IDFT_IC(i)>(up, um, s, d); },
+                      [&](auto c, const pk& r) { s2[(IDFT_IC(c) * Q2 + ln) * NV + jv] = r; });
   }
   for (int it = tid; it < N2 * NV; it += kThreads) {
     const int jv = it % NV, l2 = it / NV;  // l2 = b * N1 + a
-    {
-      const int a = l2 % N1, b = l2 / N1;
-      pk v1[N1], v2[N1];
+    const int a = l2 % N1, b = l2 / N1;
+    pk v1[N1], v2[N1], p1[(N1 + 1) / 2], m1[N1 / 2], p2[(N1 + 1) / 2], m2[N1 / 2];
 #pragma unroll
-      for (int c = 0; c < N1; ++c) {
-        v1[c] = k1[(c * N2 + l2) * NV + jv];
-        v2[c] = k2[(c * N2 + l2) * NV + jv];
-      }
-      static_for<0, N1>([&](auto c2) {
-        pk s;
-        pack_zero(s);
-        static_for<0, N1>([&](auto c) {
-          constexpr int I = IDFT_IC(c2) * N1 + IDFT_IC(c);
-          pack_fma(s, sfD<T, P, I>(), v1[IDFT_IC(c)]);
-          pack_fma(s, sfM<T, P, I>(), v2[IDFT_IC(c)]);
-        });
-        yk[((IDFT_IC(c2) * N1 + b) * N1P + a) * NV + jv] = s;
-      });
+    for (int c = 0; c < N1; ++c) {
+      v1[c] = k1[(c * N2 + l2) * NV + jv];
+      v2[c] = k2[(c * N2 + l2) * NV + jv];
     }
+    eo_split(v1, p1, m1);
+    eo_split(v2, p2, m2);
+    eo_out<N1, T, VC>(
+        [&](auto i, pk& s, pk& d) {
+          eo_row<T, TD, IDFT_IC(i)>(p1, m1, s, d);
+          eo_row<T, TM, IDFT_IC(i)>(p2, m2, s, d);
+        },
+        [&](auto c2, const pk& r) { yk[((IDFT_IC(c2) * N1 + b) * N1P + a) * NV + jv] = r; });
   }
   __syncthreads();
   // stage 4: y lines back (c, qx)
   for (int it = tid; it < N1 * Q * NV; it += kThreads) {
     const int jv = it % NV, ln = it / NV;
     const int qx = ln % Q, c = ln / Q;
-    pk v[Q];
+    pk v[Q], vp[(Q + 1) / 2], vm[Q / 2];
 #pragma unroll
     for (int qy = 0; qy < Q; ++qy) v[qy] = s2[((c * Q + qy) * Q + qx) * NV + jv];
-    static_for<0, N1>([&](auto b) {
-      pk s;
-      pack_zero(s);
-      static_for<0, Q>([&](auto qy) {
-        pack_fma(s, sfL<T, P, IDFT_IC(qy) * N1 + IDFT_IC(b)>(), v[IDFT_IC(qy)]);
-      });
-      s1[((c * N1 + IDFT_IC(b)) * QP + qx) * NV + jv] = s;
-    });
+    eo_split(v, vp, vm);
+    eo_out<N1, T, VC>(
+        [&](auto i, pk& s, pk& d) { eo_row<T, TLt, IDFT_IC(i)>(vp, vm, s, d); },
+        [&](auto b, const pk& r) { s1[((c * N1 + IDFT_IC(b)) * QP + qx) * NV + jv] = r; });
   }
   __syncthreads();
   // stage 5: x lines back (c, b) plus the kinetic part, into the cell scratch
   for (int it = tid; it < N2 * NV; it += kThreads) {
     const int jv = it % NV, ln = it / NV;
     if (jv * VC >= bw) continue;
-    pk v[Q];
+    pk v[Q], vp[(Q + 1) / 2], vm[Q / 2];
 #pragma unroll
     for (int qx = 0; qx < Q; ++qx) v[qx] = s1[(ln * QP + qx) * NV + jv];
+    eo_split(v, vp, vm);
     T* yp = Yc + (size_t)((e * NE + ln * N1) * ub + c0 + jv * VC);
-    static_for<0, N1>([&](auto a) {
-      const pk kin = yk[(ln * N1P + IDFT_IC(a)) * NV + jv];
-      pk s;
+    eo_out<N1, T, VC>([&](auto i, pk& s, pk& d) { eo_row<T, TLt, IDFT_IC(i)>(vp, vm, s, d); },
+                      [&](auto a, const pk& r) {
+                        const pk kin = yk[(ln * N1P + IDFT_IC(a)) * NV + jv];
 #pragma unroll
-      for (int u = 0; u < VC; ++u) s.v[u] = hq * kin.v[u];
-      static_for<0, Q>([&](auto qx) {
-        pack_fma(s, sfL<T, P, IDFT_IC(qx) * N1 + IDFT_IC(a)>(), v[IDFT_IC(qx)]);
-      });
-#pragma unroll
-      for (int u = 0; u < VC; ++u)
-        if (jv * VC + u < bw) yp[(size_t)IDFT_IC(a) * ub + u] = s.v[u];
-    });
+                        for (int u = 0; u < VC; ++u)
+                          if (jv * VC + u < bw) yp[(size_t)IDFT_IC(a) * ub + u] = hq * kin.v[u] + r.v[u];
+                      });
   }
 }
 #undef IDFT_IC

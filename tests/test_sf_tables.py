@@ -72,3 +72,16 @@ def test_mass_and_stiffness(tables, p):
     assert np.max(np.abs(tables[(p, "D")].reshape(n1, n1) - D)) < 1e-12 * np.max(np.abs(D))
     assert abs(M.sum() - 2.0) < 1e-13                      # integral of 1 over [-1, 1]
     assert np.max(np.abs(tables[(p, "D")].reshape(n1, n1).sum(axis=1))) < 1e-12  # D 1 = 0
+
+
+@pytest.mark.parametrize("p", [2, 3, 4, 5])
+def test_tables_centro_symmetric(tables, p):
+    """The kernel applies every 1D table in even-odd form: output R-1-i is
+    taken as e_i - o_i, which equals row R-1-i of the product only when
+    A[R-1-i][C-1-j] = A[i][j] (nodes and Gauss points symmetric about 0)."""
+    n1, q = p + 1, p + 3
+    for name, shape in (("L", (q, n1)), ("M", (n1, n1)), ("D", (n1, n1))):
+        A = tables[(p, name)].reshape(shape)
+        assert np.max(np.abs(A[::-1, ::-1] - A)) < 1e-14 * max(1.0, np.max(np.abs(A))), name
+        # a wrong pairing (reversing one axis only) is far from symmetric
+        assert np.max(np.abs(A[::-1, :] - A)) > 1e-3, name
