@@ -31,6 +31,7 @@ sys.path.insert(0, HERE)
 METRIC = "s per BFGS iter (eigen+adjoint+force), mixed vs FP64 on GPU, force err ≤1e-6"
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3  # MI355X published FP32 vector / dense matrix peak
+FP64_PEAK_TFLOPS = 78.6   # MI355X published FP64 vector / dense matrix peak
 
 
 def sf_flops_per_cell_column(p):
@@ -66,6 +67,7 @@ def sf_flops_per_cell_column(p):
 # and profiles/cell_gemm_inverse_24c_p5_pmc.md (2 16 4 0 and 1 16 0 1).
 MEASURED_TRAFFIC = {
     ("inverse_24c_p5", "k_cell_sf<float,float,5,4,1,false>"): 276.3e6,
+    ("inverse_24c_p5", "k_cell_sf<double,double,5,4,1,false>"): 552.5e6,
     ("inverse_24c_p5", "k_cell_gemm<float,float,5,1>"): 2613e6,
     ("inverse_24c_p5", "k_cell_symm<float,float,5,1>"): 1691e6,
 }
@@ -333,8 +335,11 @@ def main():
         force_err = float(np.linalg.norm(fm - ff) / np.linalg.norm(ff))
         log(f"force rel err mixed vs FP64 (tight solves): {force_err:.3e}; loss {lm:.8e} vs {lf:.8e}")
 
-    # roofline of the dominant kernel, the FP32 cell product (phase 1 of the
-    # operator), for the kernel the autotuner chose:
+    # roofline of the dominant kernel: the FP64 cell product (phase 1 of the
+    # operator) of the mixed run, which applies the operator in the MINRES steps
+    # before the switch to FP32 and takes the largest share of the iteration's
+    # GPU time; the FP32 cell product of the Chebyshev filter is reported beside
+    # it.  For the kernel the autotuner chose:
     #  - stored-H kernels (k_cell_gemm, k_cell_symm) are HBM-bound; their
     #    algorithmic bytes per launch are the symmetric H_e (ne(ne+1)/2 entries
     #    per cell) read once, the block X and node scale S read once and the
@@ -342,6 +347,7 @@ def main():
     #  - the sum-factorised kernel (k_cell_sf) is bound by the vector ALUs; its
     #    algorithmic work per launch is the 1D contractions, sf_flops(p) per
     #    cell and column, and it moves w_e + X + S + the per-cell products.
+    # The timed launches are plain applications (no node scale on the input).
     s = cm.sizes
     B = prob.desc.n_eig
     p = prob.desc.degree
@@ -355,38 +361,41 @@ def main():
     def sf_bytes(sz):
         return s.n_cells * nq * sz + s.n_nodes * (B + 1) * sz + s.n_cells * ne * B * sz
 
-    def sf_flops():
-        return s.n_cells * B * sf_flops_per_cell_column(p)
+    def roofline(shape, fp32, ms_cell, ms_op):
+        sz, tn = (4, "float") if fp32 else (8, "double")
+        kern = shape["kernel"]
+        if kern == abi.KERNEL_SUM_FACTOR:
+            kname = (f"k_cell_sf<{tn},{tn},{p},{shape['vectors_per_tile']},"
+                     f"{shape['cells_per_wg']},false>")
+            fl = s.n_cells * B * sf_flops_per_cell_column(p)
+            peak = FP32_PEAK_TFLOPS if fp32 else FP64_PEAK_TFLOPS
+            tf = fl / (ms_cell * 1e-3) / 1e12
+            r = {"bound": "valu", "achieved": round(tf, 2), "peak": peak,
+                 "unit": "TFLOP/s", "frac": round(tf / peak, 4),
+                 "traffic": MEASURED_TRAFFIC.get((args.config, kname)), "kernel": kname,
+                 "ms_per_launch": round(ms_cell, 4), "flops_per_launch": fl,
+                 "bytes_per_launch": sf_bytes(sz),
+                 "achieved_GBs": round(sf_bytes(sz) / (ms_cell * 1e-3) / 1e9, 1)}
+        else:
+            nct = min(5, max(1, min(shape["vectors_per_tile"], (B + 15) // 16 * 16) // 16))
+            kname = f"k_cell_{('gemm', 'symm')[kern]}<{tn},{tn},{p},{nct}>"
+            achieved = gemm_bytes(sz) / (ms_cell * 1e-3) / 1e9
+            r = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                 "traffic": MEASURED_TRAFFIC.get((args.config, kname)), "kernel": kname,
+                 "ms_per_launch": round(ms_cell, 4), "bytes_per_launch": gemm_bytes(sz)}
+        r["operator_ms"] = round(ms_op, 4)
+        return r
 
-    ms_op, ms_gemm = cm.profile_apply(B, abi.MIXED, reps=20)
-    tuned32 = mixed["tuned"]["fp32"]
-    kern = tuned32["kernel"]
-    if kern == abi.KERNEL_SUM_FACTOR:
-        # the plain application's instantiation (no node scale on the input)
-        kname = (f"k_cell_sf<float,float,{p},{tuned32['vectors_per_tile']},"
-                 f"{tuned32['cells_per_wg']},false>")
-        fl = sf_flops()
-        tf = fl / (ms_gemm * 1e-3) / 1e12
-        roof = {"bound": "valu", "achieved": round(tf, 2), "peak": FP32_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": round(tf / FP32_PEAK_TFLOPS, 4),
-                "traffic": MEASURED_TRAFFIC.get((args.config, kname)), "kernel": kname,
-                "ms_per_launch": round(ms_gemm, 4), "flops_per_launch": fl,
-                "bytes_per_launch": sf_bytes(4),
-                "achieved_GBs": round(sf_bytes(4) / (ms_gemm * 1e-3) / 1e9, 1)}
-    else:
-        nct = min(5, max(1, min(tuned32["vectors_per_tile"], (B + 15) // 16 * 16) // 16))
-        kname = f"k_cell_{('gemm', 'symm')[kern]}<float,float,{p},{nct}>"
-        achieved = gemm_bytes(4) / (ms_gemm * 1e-3) / 1e9
-        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": MEASURED_TRAFFIC.get((args.config, kname)), "kernel": kname,
-                "ms_per_launch": round(ms_gemm, 4), "bytes_per_launch": gemm_bytes(4)}
-    roof["operator_ms"] = round(ms_op, 4)
-    extra = {}
+    ms_op64, ms_cell64 = cm.profile_apply(B, abi.FP64, reps=20)
+    roof = roofline(mixed["tuned"]["fp64"], False, ms_cell64, ms_op64)
+    ms_op32, ms_cell32 = cm.profile_apply(B, abi.MIXED, reps=20)
+    extra = {"fp32_cell_product": roofline(mixed["tuned"]["fp32"], True, ms_cell32, ms_op32)}
     if cf is not None:
         m64, g64 = cf.profile_apply(B, abi.FP64, reps=20)
-        extra["fp64_cell_product"] = {"ms_per_launch": round(g64, 4), "operator_ms": round(m64, 4),
-                                      "shape": fp64["tuned"]["fp64"]}
+        extra["fp64_run_cell_product"] = {"ms_per_launch": round(g64, 4),
+                                          "operator_ms": round(m64, 4),
+                                          "shape": fp64["tuned"]["fp64"]}
     cpu = None
     if not args.no_cpu:
         log("CPU baseline sample (oracle)")
