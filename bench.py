@@ -60,7 +60,7 @@ def sf_flops_per_cell_column(p):
     return x_lines + y_lines + z_lines + y_back + x_back
 
 
-# HBM bytes per launch of the dominant kernel from two rocprofv3 --pmc runs
+# HBM bytes per launch of the cell-product kernels from two rocprofv3 --pmc runs
 # (FETCH_SIZE, WRITE_SIZE) of `bench.py --kernels-only --shape ...`, FETCH_SIZE
 # scaled by the copy calibration (x1.926), WRITE_SIZE as read; the summaries
 # with the commands are profiles/cell_sf_inverse_24c_p5_pmc.md (shape 1 4 4 2)
