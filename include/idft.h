@@ -66,7 +66,7 @@ enum { IDFT_LS_SECANT_LOSS = 0, IDFT_LS_SECANT_FORCE_NORM = 1, IDFT_LS_CP = 2 };
 
 typedef struct idft_desc {
   int32_t n_cells;            /* parent cells per axis                          */
-  int32_t degree;             /* p, 2..6                                        */
+  int32_t degree;             /* p, 2..5                                        */
   double half_width;          /* L                                              */
   int32_t n_eig;              /* Kohn-Sham states solved (occupied + buffer)    */
   double n_electrons;         /* N_e (spin-unpolarised: 2 per orbital)          */
@@ -87,7 +87,15 @@ typedef struct idft_desc {
   double rho_tol_constraints; /* rhoTolForConstraints (.cpp:413)                */
   int32_t gradient_mode;      /* IDFT_GRADIENT_*                                */
   int32_t lanczos_steps;      /* steps of the spectrum upper-bound Lanczos      */
-  int32_t refine_every;       /* reserved (no effect; keeps the layout)         */
+  int32_t psi_checkpoint;     /* HIP: 0 keeps the FP64 eigenvector block resident
+                                 between the forward and the backward pass; 1
+                                 keeps only an FP32 copy of it (the checkpoint),
+                                 frees the eigensolver's work buffers, and
+                                 recomputes the FP64 block in the backward pass
+                                 by one Chebyshev filter pass, orthonormalisation
+                                 and Rayleigh-Ritz from the checkpoint (less
+                                 device memory, one extra filter pass).  The
+                                 oracle keeps its vectors either way.         */
   int32_t check_every;        /* MINRES steps between host convergence checks   */
   uint64_t seed;              /* seed of the initial random subspace            */
   int32_t minres_graph;       /* HIP: replay check_every MINRES steps as one
@@ -124,6 +132,9 @@ typedef struct idft_stats {
   int64_t applies_fp32;
   int64_t apply_cols_fp64;    /* sum of block widths over those applications    */
   int64_t apply_cols_fp32;
+  int32_t adjoint_blocks;     /* blocks of the last backward pass after the
+                                 degeneracy-preserving shrink (ISF:930-955)   */
+  int32_t psi_recomputes;     /* checkpoint recomputations (cumulative)       */
 } idft_stats;
 
 typedef struct idft_ctx idft_ctx;
@@ -188,6 +199,10 @@ int idft_backward(idft_ctx* ctx, double* force);
 int idft_set_potential(idft_ctx* ctx, const double* pot);
 /* Set v_KS at parent quadrature directly and rebuild H_e. */
 int idft_set_veff_quad(idft_ctx* ctx, const double* veff_q);
+/* The v_KS of the current Hamiltonian at parent quadrature [cells*Q^3]
+ * (potBase + interpolated v_xc after idft_set_potential; the reference forms
+ * it at InverseDFTSolverFunction.cpp:1360-1381). */
+int idft_get_veff_quad(const idft_ctx* ctx, double* veff_q);
 /* ChFSI passes on the current Hamiltonian until the max residual over the
  * occupied states is below tol or max_passes is reached
  * (InverseDFTSolverFunction.cpp:1390-1474). */
@@ -197,6 +212,9 @@ int idft_eigensolve(idft_ctx* ctx, double tol, int32_t max_passes,
 int idft_get_eigenvectors(const idft_ctx* ctx, double* psi);
 int idft_set_eigenvectors(idft_ctx* ctx, const double* psi);
 int idft_get_occupancies(const idft_ctx* ctx, double* occ, double* fermi);
+/* Ritz values of the current eigenvector block [n_eig] (after the last
+ * Rayleigh-Ritz; IDFT_ESTATE before any). */
+int idft_get_eigenvalues(const idft_ctx* ctx, double* eigenvalues);
 /* Total density rho = 2 sum_i f_i psi_i^2 at parent quadrature [cells*Q^3]. */
 int idft_get_density(idft_ctx* ctx, double* rho_q);
 /* y = M^{-1/2} H M^{-1/2} x (Loewdin Hamiltonian), boundary rows/cols zero.

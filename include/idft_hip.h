@@ -21,8 +21,8 @@ int idft_hip_init(void);
  * columns, FP64 or FP32 (precision = IDFT_FP64 / IDFT_MIXED), over reps
  * repetitions after 2 warm-up applications, taking three different random
  * blocks in turn so that the input is not cache-resident: ms_per_apply for the whole
- * operator (MFMA cell-product kernel + node-gather kernel), ms_gemm (may be
- * NULL) for the MFMA cell-product kernel alone.  No counterpart in the
+ * operator (cell-product kernel + node-gather kernel), ms_gemm (may be
+ * NULL) for the cell-product kernel alone.  No counterpart in the
  * reference; used by the benchmark to measure the dominant kernel live. */
 int idft_profile_apply(idft_ctx* ctx, int32_t ncols, int32_t precision, int32_t reps,
                        double* ms_per_apply, double* ms_gemm);
@@ -65,6 +65,23 @@ int idft_set_apply_shape(idft_ctx* ctx, int32_t precision, int32_t cells_per_wg,
 int idft_autotune_apply(idft_ctx* ctx, int32_t ncols, int32_t precision, int32_t reps,
                         int32_t* cells_per_wg, int32_t* vectors_per_tile, int32_t* kernel,
                         int32_t* gather_rows, double* ms_best);
+
+/* Multivector handle over the caller's device memory: ptr holds n_nodes *
+ * ncols doubles, node-major ([node][col]), on the device of the context, and
+ * must outlive the handle; idft_mv_destroy releases the handle only.  With
+ * it a caller whose multivectors already live in device memory (DFT-FE's
+ * MultiVector on the device, MultiVectorAdjointLinearSolverProblem.h:41-88)
+ * drives the contract-2 entry points without a host copy.  No counterpart
+ * in the reference (its multivectors are the solver's own objects). */
+int idft_mv_wrap(idft_ctx* ctx, double* device_ptr, int32_t ncols, idft_mv** out);
+/* Device pointer of a multivector handle (owned or wrapped). */
+double* idft_mv_device_ptr(const idft_mv* mv);
+
+/* Device memory held by this library's allocations in the process: current
+ * bytes and the peak since the last reset (reset_peak != 0 restarts the
+ * peak at the current value after reading).  No counterpart in the
+ * reference; used to report the peak HBM of a run. */
+int idft_device_memory(int64_t* current_bytes, int64_t* peak_bytes, int32_t reset_peak);
 
 #ifdef __cplusplus
 }

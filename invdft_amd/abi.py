@@ -31,7 +31,7 @@ class Desc(C.Structure):
         ("degeneracy_tol", C.c_double), ("frac_occ_tol", C.c_double),
         ("additional_states", C.c_int32), ("block_size", C.c_int32),
         ("rho_tol_constraints", C.c_double), ("gradient_mode", C.c_int32),
-        ("lanczos_steps", C.c_int32), ("refine_every", C.c_int32), ("check_every", C.c_int32),
+        ("lanczos_steps", C.c_int32), ("psi_checkpoint", C.c_int32), ("check_every", C.c_int32),
         ("seed", C.c_uint64), ("minres_graph", C.c_int32), ("relax_ratio", C.c_double),
     ]
 
@@ -53,7 +53,8 @@ class Stats(C.Structure):
                 ("l1_density_error", C.c_double), ("int_rho_veff", C.c_double),
                 ("n_evaluations", C.c_int64), ("spectrum_upper_bound", C.c_double),
                 ("applies_fp64", C.c_int64), ("applies_fp32", C.c_int64),
-                ("apply_cols_fp64", C.c_int64), ("apply_cols_fp32", C.c_int64)]
+                ("apply_cols_fp64", C.c_int64), ("apply_cols_fp32", C.c_int64),
+                ("adjoint_blocks", C.c_int32), ("psi_recomputes", C.c_int32)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -82,10 +83,12 @@ _PROTOS = {
     "idft_backward": (C.c_int, [C.c_void_p, _DP]),
     "idft_set_potential": (C.c_int, [C.c_void_p, _DP]),
     "idft_set_veff_quad": (C.c_int, [C.c_void_p, _DP]),
+    "idft_get_veff_quad": (C.c_int, [C.c_void_p, _DP]),
     "idft_eigensolve": (C.c_int, [C.c_void_p, C.c_double, C.c_int32, _DP, _DP, _IP]),
     "idft_get_eigenvectors": (C.c_int, [C.c_void_p, _DP]),
     "idft_set_eigenvectors": (C.c_int, [C.c_void_p, _DP]),
     "idft_get_occupancies": (C.c_int, [C.c_void_p, _DP, _DP]),
+    "idft_get_eigenvalues": (C.c_int, [C.c_void_p, _DP]),
     "idft_get_density": (C.c_int, [C.c_void_p, _DP]),
     "idft_apply_hamiltonian": (C.c_int, [C.c_void_p, _DP, _DP, C.c_int32, C.c_int32]),
     "idft_chebyshev_filter": (C.c_int, [C.c_void_p, _DP, C.c_int32, C.c_int32, C.c_double,
@@ -331,6 +334,11 @@ class Context:
     def set_veff_quad(self, v):
         self._check(self.lib.idft_set_veff_quad(self.h, _dp(_f64(v))))
 
+    def veff_quad(self):
+        out = np.zeros(self.nq_total)
+        self._check(self.lib.idft_get_veff_quad(self.h, _dp(out)))
+        return out
+
     def eigensolve(self, tol, max_passes=100):
         ev = np.zeros(self.desc.n_eig)
         mr = np.zeros(1)
@@ -352,6 +360,11 @@ class Context:
         ef = np.zeros(1)
         self._check(self.lib.idft_get_occupancies(self.h, _dp(occ), _dp(ef)))
         return occ, float(ef[0])
+
+    def eigenvalues(self):
+        ev = np.zeros(self.desc.n_eig)
+        self._check(self.lib.idft_get_eigenvalues(self.h, _dp(ev)))
+        return ev
 
     def density(self):
         out = np.zeros(self.nq_total)

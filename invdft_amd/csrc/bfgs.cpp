@@ -19,6 +19,11 @@ using Vec = std::vector<double>;
 
 const char kMagic[8] = {'I', 'D', 'F', 'T', 'L', 'B', 'F', '1'};
 
+// A failing dotProduct inside the iteration, carried to idft_bfgs_solve's status.
+struct DotFailure {
+  int rc;
+};
+
 class Lbfgs {
  public:
   Lbfgs(idft_ctx* c, int64_t n, int history) : c_(c), n_(n), history_(history) {}
@@ -155,7 +160,8 @@ class Lbfgs {
 
   double dot(const Vec& a, const Vec& b) {
     double o = 0.0;
-    idft_dot(c_, a.data(), b.data(), n_, 1, &o);
+    const int rc = idft_dot(c_, a.data(), b.data(), n_, 1, &o);
+    if (rc != IDFT_OK) throw DotFailure{rc};
     return o;
   }
 
@@ -272,6 +278,8 @@ extern "C" int idft_bfgs_solve(idft_ctx* c, int32_t ls_type, int32_t max_iter, d
     Lbfgs solver(c, sz.n_vxc, history);
     return solver.run(ls_type, max_iter, tol, ls_tol, n_line_search, loss_hist, gnorm_hist,
                       time_hist, n_done);
+  } catch (const DotFailure& f) {
+    return f.rc;
   } catch (const std::exception&) {
     return IDFT_ENOMEM;
   }

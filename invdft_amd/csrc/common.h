@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <stdexcept>
 #include <string>
@@ -27,6 +28,23 @@ struct Error : std::runtime_error {
       throw ::idft::Error(IDFT_EHIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
+// Device bytes held by DevBuf allocations of this library, process-wide: the
+// current total and its peak since the last reset (idft_device_memory).
+inline std::atomic<int64_t>& devmem_current() {
+  static std::atomic<int64_t> v{0};
+  return v;
+}
+inline std::atomic<int64_t>& devmem_peak() {
+  static std::atomic<int64_t> v{0};
+  return v;
+}
+inline void devmem_add(int64_t bytes) {
+  const int64_t now = devmem_current().fetch_add(bytes) + bytes;
+  int64_t pk = devmem_peak().load();
+  while (now > pk && !devmem_peak().compare_exchange_weak(pk, now)) {
+  }
+}
+
 // Owning device buffer (hipMalloc'd, freed on destruction).
 template <typename T>
 struct DevBuf {
@@ -43,7 +61,10 @@ struct DevBuf {
   }
   ~DevBuf() { release(); }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) {
+      (void)hipFree(p);
+      devmem_add(-(int64_t)(n * sizeof(T)));
+    }
     p = nullptr; n = 0;
   }
   // Grow-only reallocation; contents are not preserved.
@@ -54,6 +75,7 @@ struct DevBuf {
     hipError_t e = hipMalloc(&p, count * sizeof(T));
     if (e != hipSuccess) { p = nullptr; throw Error(IDFT_ENOMEM, "hipMalloc failed"); }
     n = count;
+    devmem_add((int64_t)(n * sizeof(T)));
   }
   void upload(const T* h, size_t count, hipStream_t s) {
     resize(count);

@@ -21,7 +21,9 @@ using namespace idft;
 struct idft_mv {
   idft_ctx* ctx = nullptr;
   int ncols = 0;
-  DevBuf<double> d;
+  DevBuf<double> own;     // storage of idft_mv_create
+  double* ext = nullptr;  // caller's device storage (idft_mv_wrap), not freed here
+  double* p() const { return ext ? ext : own.p; }
 };
 
 namespace {
@@ -659,7 +661,7 @@ struct idft_ctx {
 
   void compute_mu(const std::vector<double>& occv) {
     const int Bb = adj.B;
-    std::vector<double> v = pair_quad_sum(adj.psi->d.p, adj.psi->d.p, Bb, adj.uJxW.p, t.nq);
+    std::vector<double> v = pair_quad_sum(adj.psi->p(), adj.psi->p(), Bb, adj.uJxW.p, t.nq);
     adj.Mu.assign((size_t)Bb * Bb, 0.0);
     for (size_t k = 0; k < v.size(); ++k) {
       const int i = adj.pairs[2 * k], j = adj.pairs[2 * k + 1];
@@ -673,7 +675,7 @@ struct idft_ctx {
     if (!psi || psi->ncols != b || b <= 0) throw Error(IDFT_EINVAL, "psi block size mismatch");
     AdjointState& A = adj;
     A.B = b; A.psi = psi; A.fermi = fermiE; A.x = nullptr;
-    zero_boundary(psi->d.p, b);
+    zero_boundary(psi->p(), b);
     A.occ.assign(occv, occv + b);
     std::vector<double> occ4(b);
     for (int i = 0; i < b; ++i) occ4[i] = 4.0 * occv[i];
@@ -703,7 +705,7 @@ struct idft_ctx {
     const int b = A.B;
     if (x && x->ncols != b) throw Error(IDFT_EINVAL, "block size mismatch");
     A.x = x;
-    const double* psi = A.psi->d.p;
+    const double* psi = A.psi->p();
     compute_mu(A.occ);
     // rhs = -2 sqrt(M) psi Mu   (column i: -2 sum_k Mu[k][i] M^{1/2} psi_k)
     std::vector<double> C((size_t)b * b);
@@ -738,8 +740,8 @@ struct idft_ctx {
     AdjointState& A = adj;
     if (!A.ready || !A.x) throw Error(IDFT_ESTATE, "distributeX before computeRhs");
     const int b = A.B;
-    double* x = A.x->d.p;
-    const double* psi = A.psi->d.p;
+    double* x = A.x->p();
+    const double* psi = A.psi->p();
     launch_lincomb<double>(t.N, b, x, 0.0, x, 1.0, nullptr, nullptr, 0.0, nullptr, invSqrtM.p, s);
     zero_boundary(x, b);
     const int np = (int)A.pairs.size() / 2;
@@ -922,7 +924,7 @@ struct idft_ctx {
       for (double v : nr)
         if (!std::isfinite(v)) throw Error(IDFT_ENAN, "non-finite adjoint rhs");
     }
-    double* xp = x->d.p;
+    double* xp = x->p();
     launch_lincomb<double>(t.N, b, xp, 0.0, nullptr, 0.0, nullptr, nullptr, 0.0, nullptr, nullptr, s);
     std::vector<int> total(b, 0);
     if (!mixed) {
@@ -1035,7 +1037,9 @@ struct idft_ctx {
     double tForce = 0.0;
     const int defBS = d.block_size > 0 ? d.block_size : B;
     int jvec = 0;
+    st.adjoint_blocks = 0;
     while (jvec < B) {
+      st.adjoint_blocks++;
       int bs = std::min(defBS, B - jvec);
       while (true) {
         const int last = jvec + bs - 1;
@@ -1043,8 +1047,8 @@ struct idft_ctx {
         else break;
       }
       mvPsi.ncols = bs; mvPsi.ctx = this;
-      mvPsi.d.resize(NB(bs));
-      launch_copy_cols(t.N, B, jvec, bs, psiAll.p, mvPsi.d.p, s);
+      mvPsi.own.resize(NB(bs));
+      launch_copy_cols(t.N, B, jvec, bs, psiAll.p, mvPsi.p(), s);
       std::vector<double> occb(bs), shifts(bs);
       for (int j = 0; j < bs; ++j) { shifts[j] = eig[jvec + j]; occb[j] = occ[jvec + j]; }
       // degeneracy map (InverseDFTSolverFunction.cpp:37-77)
@@ -1067,14 +1071,14 @@ struct idft_ctx {
       adjointTol = std::min(adjointTol, tolCheb / d.adaptive_factor_adjoint);
       st.adjoint_tol_used = adjointTol;
       mvX.ncols = bs; mvX.ctx = this;
-      mvX.d.resize(NB(bs));
+      mvX.own.resize(NB(bs));
       minres(&mvX, adjointTol, d.adjoint_max_iter, 1, nullptr);
-      launch_zero_rows(t.N, bs, mvX.d.p, adjMask.p, s);
+      launch_zero_rows(t.N, bs, mvX.p(), adjMask.p, s);
       const double tf = wall_ms();
       const int mode = d.gradient_mode == IDFT_GRADIENT_REFERENCE ? 0 : 1;
       const double jc = std::pow(0.25 * t.h, 3);
       for (int colour = 0; colour < 8; ++colour)
-        launch_force(g, dt, mvPsi.d.p, mvX.d.p, bs, mode, jc, jxw.p, force.p, colour, s);
+        launch_force(g, dt, mvPsi.p(), mvX.p(), bs, mode, jc, jxw.p, force.p, colour, s);
       sync();
       tForce += wall_ms() - tf;
       jvec += bs;
@@ -1282,6 +1286,23 @@ int idft_set_veff_quad(idft_ctx* c, const double* vq) {
   });
 }
 
+int idft_get_eigenvalues(const idft_ctx* c, double* ev) {
+  if (!c || !ev) return IDFT_EINVAL;
+  if (!c->haveRitz || c->eig.empty()) return IDFT_ESTATE;
+  std::copy(c->eig.begin(), c->eig.end(), ev);
+  return IDFT_OK;
+}
+
+int idft_get_veff_quad(const idft_ctx* cc, double* vq) {
+  idft_ctx* c = const_cast<idft_ctx*>(cc);
+  if (!c || !vq) return IDFT_EINVAL;
+  if (!c->haveH) return IDFT_ESTATE;
+  return guarded(c, [&] {
+    c->veff.download(vq, c->nQ, c->s);
+    c->sync();
+  });
+}
+
 int idft_eigensolve(idft_ctx* c, double tol, int32_t maxp, double* ev, double* mr, int32_t* passes) {
   if (!c) return IDFT_EINVAL;
   return guarded(c, [&] {
@@ -1379,12 +1400,36 @@ int idft_mv_create(idft_ctx* c, int32_t ncols, idft_mv** out) {
     idft_mv* v = new idft_mv();
     v->ctx = c;
     v->ncols = ncols;
-    v->d.resize(c->NB(ncols));
-    launch_lincomb<double>(c->t.N, ncols, v->d.p, 0.0, nullptr, 0.0, nullptr, nullptr, 0.0, nullptr,
+    v->own.resize(c->NB(ncols));
+    launch_lincomb<double>(c->t.N, ncols, v->p(), 0.0, nullptr, 0.0, nullptr, nullptr, 0.0, nullptr,
                            nullptr, c->s);
     c->sync();
     *out = v;
   });
+}
+
+int idft_mv_wrap(idft_ctx* c, double* dptr, int32_t ncols, idft_mv** out) {
+  if (!c || !dptr || !out || ncols < 1 || ncols > kMaxCols) return IDFT_EINVAL;
+  hipPointerAttribute_t attr;
+  if (hipPointerGetAttributes(&attr, dptr) != hipSuccess || attr.type != hipMemoryTypeDevice) {
+    c->err = "idft_mv_wrap: not a device pointer";
+    return IDFT_EINVAL;
+  }
+  idft_mv* v = new idft_mv();
+  v->ctx = c;
+  v->ncols = ncols;
+  v->ext = dptr;
+  *out = v;
+  return IDFT_OK;
+}
+
+double* idft_mv_device_ptr(const idft_mv* v) { return v ? v->p() : nullptr; }
+
+int idft_device_memory(int64_t* cur, int64_t* peak, int32_t reset) {
+  if (cur) *cur = devmem_current().load();
+  if (peak) *peak = devmem_peak().load();
+  if (reset) devmem_peak().store(devmem_current().load());
+  return IDFT_OK;
 }
 
 void idft_mv_destroy(idft_mv* v) { delete v; }
@@ -1392,7 +1437,8 @@ void idft_mv_destroy(idft_mv* v) { delete v; }
 int idft_mv_set(idft_ctx* c, idft_mv* v, const double* h) {
   if (!c || !v || !h) return IDFT_EINVAL;
   return guarded(c, [&] {
-    v->d.upload(h, c->NB(v->ncols), c->s);
+    IDFT_HIP(hipMemcpyAsync(v->p(), h, c->NB(v->ncols) * sizeof(double), hipMemcpyHostToDevice,
+                            c->s));
     c->sync();
   });
 }
@@ -1400,7 +1446,8 @@ int idft_mv_set(idft_ctx* c, idft_mv* v, const double* h) {
 int idft_mv_get(idft_ctx* c, const idft_mv* v, double* h) {
   if (!c || !v || !h) return IDFT_EINVAL;
   return guarded(c, [&] {
-    v->d.download(h, c->NB(v->ncols), c->s);
+    IDFT_HIP(hipMemcpyAsync(h, v->p(), c->NB(v->ncols) * sizeof(double), hipMemcpyDeviceToHost,
+                            c->s));
     c->sync();
   });
 }
@@ -1420,7 +1467,7 @@ int idft_adj_compute_rhs(idft_ctx* c, idft_mv* ndbc, idft_mv* x, idft_mv* rhs) {
   (void)ndbc;
   return guarded(c, [&] {
     if (rhs->ncols != c->adj.B) throw Error(IDFT_EINVAL, "block size mismatch");
-    c->compute_rhs(x, rhs->d.p);
+    c->compute_rhs(x, rhs->p());
   });
 }
 
@@ -1429,7 +1476,7 @@ int idft_adj_vmult(idft_ctx* c, idft_mv* Ax, idft_mv* x, int32_t block) {
   return guarded(c, [&] {
     if (block != c->adj.B || x->ncols != block || Ax->ncols != block)
       throw Error(IDFT_EINVAL, "block size mismatch");
-    c->vmult(Ax->d.p, x->d.p, block);
+    c->vmult(Ax->p(), x->p(), block);
     c->sync();
   });
 }
@@ -1437,7 +1484,7 @@ int idft_adj_vmult(idft_ctx* c, idft_mv* Ax, idft_mv* x, int32_t block) {
 int idft_adj_precondition_jacobi(idft_ctx* c, idft_mv* dst, const idft_mv* src, double) {
   if (!c || !dst || !src || dst->ncols != src->ncols) return IDFT_EINVAL;
   return guarded(c, [&] {
-    launch_lincomb<double>(c->t.N, src->ncols, dst->d.p, 0.0, src->d.p, 1.0, nullptr, nullptr, 0.0,
+    launch_lincomb<double>(c->t.N, src->ncols, dst->p(), 0.0, src->p(), 1.0, nullptr, nullptr, 0.0,
                            nullptr, c->Djac.p, c->s);
     c->sync();
   });
@@ -1446,7 +1493,7 @@ int idft_adj_precondition_jacobi(idft_ctx* c, idft_mv* dst, const idft_mv* src, 
 int idft_adj_precondition_jacobi_sqrt(idft_ctx* c, idft_mv* dst, const idft_mv* src, double) {
   if (!c || !dst || !src || dst->ncols != src->ncols) return IDFT_EINVAL;
   return guarded(c, [&] {
-    launch_lincomb<double>(c->t.N, src->ncols, dst->d.p, 0.0, src->d.p, 1.0, nullptr, nullptr, 0.0,
+    launch_lincomb<double>(c->t.N, src->ncols, dst->p(), 0.0, src->p(), 1.0, nullptr, nullptr, 0.0,
                            nullptr, c->DjacSqrt.p, c->s);
     c->sync();
   });
@@ -1461,7 +1508,7 @@ int idft_adj_dot_quad_wise(idft_ctx* c, idft_mv* v1, idft_mv* v2, double* out) {
   if (!c || !v1 || !v2 || !out) return IDFT_EINVAL;
   return guarded(c, [&] {
     if (!c->adj.ready) throw Error(IDFT_ESTATE, "no adjoint problem");
-    std::vector<double> o = c->dot_quad_wise(v1->d.p, v2->d.p, c->adj.B);
+    std::vector<double> o = c->dot_quad_wise(v1->p(), v2->p(), c->adj.B);
     std::copy(o.begin(), o.end(), out);
   });
 }

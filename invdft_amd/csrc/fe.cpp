@@ -252,7 +252,7 @@ void idft_desc_default(idft_desc* d) {
   d->adjoint_initial_tol = 1e-11; d->adjoint_max_iter = 5000; d->degeneracy_tol = 0.002;
   d->frac_occ_tol = 1e-8; d->additional_states = 0; d->block_size = 0;
   d->rho_tol_constraints = 1e-6; d->gradient_mode = IDFT_GRADIENT_REFERENCE;
-  d->lanczos_steps = 12; d->refine_every = 0; d->check_every = 8; d->seed = 1;
+  d->lanczos_steps = 12; d->psi_checkpoint = 0; d->check_every = 8; d->seed = 1;
   d->minres_graph = 1; d->relax_ratio = 1e3;
 }
 

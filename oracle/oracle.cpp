@@ -1116,7 +1116,9 @@ int backward(idft_ctx* c, double* force) {
   c->st.minres_iters_max = 0; c->st.minres_iters_total = 0;
   const int defBS = c->d.block_size > 0 ? c->d.block_size : Btot;
   int jvec = 0;
+  c->st.adjoint_blocks = 0;
   while (jvec < Btot) {
+    c->st.adjoint_blocks++;
     int bs = std::min(defBS, Btot - jvec);
     while (true) {
       int last = jvec + bs - 1;
@@ -1211,7 +1213,7 @@ void idft_desc_default(idft_desc* d) {
   d->adjoint_initial_tol = 1e-11; d->adjoint_max_iter = 5000; d->degeneracy_tol = 0.002;
   d->frac_occ_tol = 1e-8; d->additional_states = 0; d->block_size = 0;
   d->rho_tol_constraints = 1e-6; d->gradient_mode = IDFT_GRADIENT_REFERENCE;
-  d->lanczos_steps = 12; d->refine_every = 0; d->check_every = 8; d->seed = 1;
+  d->lanczos_steps = 12; d->psi_checkpoint = 0; d->check_every = 8; d->seed = 1;
   d->minres_graph = 1;  // HIP launch option; no effect here
   d->relax_ratio = 1e3;  // mixed-precision option; no effect here
 }
@@ -1349,6 +1351,20 @@ int idft_set_veff_quad(idft_ctx* c, const double* vq) {
   if (!c || !vq) return IDFT_EINVAL;
   c->veff.assign(vq, vq + c->m.ncell * (long)c->m.nq);
   build_hamiltonian(c);
+  return IDFT_OK;
+}
+
+int idft_get_eigenvalues(const idft_ctx* c, double* ev) {
+  if (!c || !ev) return IDFT_EINVAL;
+  if (!c->haveRitz || c->eig.empty()) return IDFT_ESTATE;
+  std::copy(c->eig.begin(), c->eig.end(), ev);
+  return IDFT_OK;
+}
+
+int idft_get_veff_quad(const idft_ctx* c, double* vq) {
+  if (!c || !vq) return IDFT_EINVAL;
+  if (!c->haveH) return IDFT_ESTATE;
+  std::copy(c->veff.begin(), c->veff.end(), vq);
   return IDFT_OK;
 }
 

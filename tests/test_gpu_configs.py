@@ -1,0 +1,84 @@
+"""The named workloads at their full sizes on the GPU (gfx950 only).
+
+The oracle is too slow at these sizes (minutes to hours per evaluation), so
+the FP64 GPU path -- pinned to the oracle on the small meshes -- is the
+reference here, and the checks are the mixed-precision errors the workloads
+state: force error <= 1e-6 relative, with the eigenvalue and density errors
+printed.  The uneven-batch case runs small problems on concurrent contexts
+and compares each with the oracle.
+"""
+import numpy as np
+import pytest
+
+from invdft_amd import abi, batch, synthetic
+from tests.helpers import SMALL, rel
+
+pytestmark = pytest.mark.gpu
+
+
+def _mixed_vs_fp64(glib, name, seed):
+    cfg = dict(synthetic.CONFIGS[name])
+    print(f"CASE {name} set-up", flush=True)
+    prob = synthetic.make_problem(glib, seed=seed, target_tol=1e-10, **cfg)
+    kw = dict(cheb_tol=1e-10, initial_tol_cheb=1e-10)
+    out = {}
+    for prec in (abi.FP64, abi.MIXED):
+        print(f"CASE {name} getForceVector precision {prec}", flush=True)
+        c = prob.context(glib, precision=prec, **kw)
+        f, L = c.get_force(prob.vxc0)
+        out[prec] = dict(force=f, loss=L, ev=c.eigenvalues(), rho=c.density(), st=c.stats())
+        c.close()
+    m, d = out[abi.MIXED], out[abi.FP64]
+    err = dict(force=rel(m["force"], d["force"]), loss=abs(m["loss"] - d["loss"]) / abs(d["loss"]),
+               eig=float(np.max(np.abs(m["ev"] - d["ev"]))), rho_l2=rel(m["rho"], d["rho"]))
+    print(f"{name}: mixed vs FP64 force {err['force']:.3e} loss {err['loss']:.3e} "
+          f"eigenvalues {err['eig']:.3e} Ha density L2 {err['rho_l2']:.3e}; "
+          f"t_total {m['st']['t_total_ms']:.0f} / {d['st']['t_total_ms']:.0f} ms", flush=True)
+    return err
+
+
+def test_inverse_24c_p5_mixed_force_error(glib):
+    """24^3 cells, degree 5 (1.77 M nodes), 12 states: one getForceVector."""
+    err = _mixed_vs_fp64(glib, "inverse_24c_p5", 2)
+    assert err["force"] <= 1e-6
+    assert err["eig"] <= 1e-8 and err["rho_l2"] <= 1e-7
+
+
+@pytest.mark.timeout(900)
+def test_batched_40c_p4_mixed_force_error(glib):
+    """40^3 cells, degree 4 (4.17 M nodes), 64 occupied + 16 buffer states."""
+    err = _mixed_vs_fp64(glib, "batched_40c_p4", 2)
+    assert err["force"] <= 1e-6
+    assert err["eig"] <= 1e-8 and err["rho_l2"] <= 1e-7
+
+
+def test_uneven_batch_concurrent_contexts_match_oracle(glib, olib):
+    """Three problems of different sizes and block widths on concurrent GPU
+    contexts (one host thread and HIP stream each): every problem's loss
+    history equals the oracle's sequential solve."""
+    print("CASE uneven batch of 3 on concurrent contexts", flush=True)
+    probs = []
+    for k, (nocc, deg) in enumerate(((1, 3), (2, 4), (3, 5))):
+        kw = dict(SMALL, degree=deg, n_eig=nocc + 4, n_electrons=2.0 * nocc)
+        probs.append(synthetic.make_problem(olib, seed=20 + k, target_tol=1e-10, **kw))
+    tol = dict(cheb_tol=1e-10, initial_tol_cheb=1e-10)
+    hist = {}
+
+    def solve(lib, prob, prec, steps, tol_kw):
+        ctx = prob.context(lib, precision=prec, **tol_kw)
+        loss, _, n = ctx.bfgs_solve(ls_type=abi.LS_CP, max_iter=steps)
+        hist[(id(lib), id(prob))] = loss
+        ctx.close()
+        return 0.0, float(loss[0]), float(loss[-1]), n
+
+    orig = batch._solve
+    batch._solve = solve
+    try:
+        batch.run_concurrent(glib, probs, abi.FP64, 2, workers=3, tol_kw=tol)
+        batch.run_sequential(olib, probs, abi.FP64, 2, tol_kw=tol)
+    finally:
+        batch._solve = orig
+    for prob in probs:
+        lg, lo = hist[(id(glib), id(prob))], hist[(id(olib), id(prob))]
+        assert len(lg) == len(lo) == 3
+        assert np.allclose(lg, lo, rtol=1e-8, atol=0), (lg, lo)

@@ -81,9 +81,12 @@ def _adjoint_setup(ctx, prob, psi, occ, eig, u, pairs, fermi):
     return mpsi, neg
 
 
-def test_adjoint_contract(glib, olib):
-    _say("adjoint_contract")
-    prob = small_problem(olib)
+@pytest.mark.parametrize("p", [4, 5])
+def test_adjoint_contract(glib, olib, p):
+    """Every contract-2 entry point at degree 4 and 5 (the node-scaled
+    sum-factorised cell products of the MINRES steps at both)."""
+    _say("adjoint_contract", p)
+    prob = small_problem(olib, degree=p)
     o = prob.context(olib)
     o.set_potential(prob.vxc0)
     ev, _, _ = o.eigensolve(1e-11, 60)
@@ -111,15 +114,18 @@ def test_adjoint_contract(glib, olib):
         c.adj_vmult(ax, xin)
         pj = c.mv(B)
         c.adj_precondition_jacobi(pj, xin)
+        pjs = c.mv(B)
+        c.adj_precondition_jacobi_sqrt(pjs, xin)
         dq = c.adj_dot_quad_wise(mpsi, mpsi, len(pairs) // 2)
         sol = c.mv(B)
         its = c.minres_solve(sol, None, 1e-13, 3000, distribute=True)
-        res[name] = dict(rhs=rhs.get(), ax=ax.get(), pj=pj.get(), dq=dq, sol=sol.get(), its=its,
-                         keep=(mpsi, x, rhs, xin, ax, pj, sol))
+        res[name] = dict(rhs=rhs.get(), ax=ax.get(), pj=pj.get(), pjs=pjs.get(), dq=dq,
+                         sol=sol.get(), its=its, keep=(mpsi, x, rhs, xin, ax, pj, pjs, sol))
     G, O = res["g"], res["o"]
     assert rel(G["rhs"], O["rhs"]) < 1e-11
     assert rel(G["ax"], O["ax"]) < 1e-13
     assert rel(G["pj"], O["pj"]) < 1e-15
+    assert rel(G["pjs"], O["pjs"]) < 1e-15
     assert np.max(np.abs(G["dq"] - O["dq"])) < 1e-12
     assert rel(G["sol"], O["sol"]) < 1e-8
     assert np.all(np.abs(G["its"] - O["its"]) <= 3)
@@ -161,10 +167,11 @@ def test_minres_graph_replay_matches_launches(glib, olib, prec):
         assert np.array_equal(out[0][k][0], out[1][k][0])
 
 
+@pytest.mark.parametrize("p", [4, 5])
 @pytest.mark.parametrize("mode", [abi.GRADIENT_REFERENCE, abi.GRADIENT_EXACT])
-def test_get_force(glib, olib, mode):
-    _say("get_force", mode)
-    prob = small_problem(olib)
+def test_get_force(glib, olib, mode, p):
+    _say("get_force", mode, p)
+    prob = small_problem(olib, degree=p)
     kw = dict(cheb_tol=1e-11, initial_tol_cheb=1e-11, gradient_mode=mode)
     g = prob.context(glib, **kw)
     o = prob.context(olib, **kw)
@@ -174,9 +181,10 @@ def test_get_force(glib, olib, mode):
     assert rel(fg, fo) < 1e-7
 
 
-def test_mixed_precision_force(glib, olib):
-    _say("mixed_force")
-    prob = small_problem(olib)
+@pytest.mark.parametrize("p", [4, 5])
+def test_mixed_precision_force(glib, olib, p):
+    _say("mixed_force", p)
+    prob = small_problem(olib, degree=p)
     kw = dict(cheb_tol=1e-10, initial_tol_cheb=1e-10)
     gm = prob.context(glib, precision=abi.MIXED, **kw)
     o = prob.context(olib, **kw)
@@ -186,11 +194,14 @@ def test_mixed_precision_force(glib, olib):
     assert rel(fm, fo) < 1e-6
 
 
-def test_bfgs_two_iterations(glib, olib):
-    _say("bfgs")
-    prob = small_problem(olib)
+@pytest.mark.parametrize("p,prec", [(4, abi.FP64), (5, abi.MIXED)])
+def test_bfgs_two_iterations(glib, olib, p, prec):
+    """Two L-BFGS iterations against the oracle's FP64 driver, in FP64 at
+    degree 4 and in mixed precision at degree 5."""
+    _say("bfgs", p, prec)
+    prob = small_problem(olib, degree=p)
     kw = dict(cheb_tol=1e-11, initial_tol_cheb=1e-11)
-    g = prob.context(glib, **kw)
+    g = prob.context(glib, precision=prec, **kw)
     o = prob.context(olib, **kw)
     lg, gg, ng = g.bfgs_solve(max_iter=2)
     lo, go, no = o.bfgs_solve(max_iter=2)
