@@ -29,10 +29,14 @@ def _mixed_vs_fp64(glib, name, seed):
         out[prec] = dict(force=f, loss=L, ev=c.eigenvalues(), rho=c.density(), st=c.stats())
         c.close()
     m, d = out[abi.MIXED], out[abi.FP64]
+    # the eigensolver converges the occupied states (the buffer states above
+    # them are outside its residual criterion), so those are compared
+    nocc = int(round(cfg["n_electrons"] / 2))
     err = dict(force=rel(m["force"], d["force"]), loss=abs(m["loss"] - d["loss"]) / abs(d["loss"]),
-               eig=float(np.max(np.abs(m["ev"] - d["ev"]))), rho_l2=rel(m["rho"], d["rho"]))
+               eig=float(np.max(np.abs(m["ev"][:nocc] - d["ev"][:nocc]))),
+               eig_buffer=float(np.max(np.abs(m["ev"] - d["ev"]))), rho_l2=rel(m["rho"], d["rho"]))
     print(f"{name}: mixed vs FP64 force {err['force']:.3e} loss {err['loss']:.3e} "
-          f"eigenvalues {err['eig']:.3e} Ha density L2 {err['rho_l2']:.3e}; "
+          f"eigenvalues {err['eig']:.3e} Ha (with buffer states {err['eig_buffer']:.3e}) density L2 {err['rho_l2']:.3e}; "
           f"t_total {m['st']['t_total_ms']:.0f} / {d['st']['t_total_ms']:.0f} ms", flush=True)
     return err
 
