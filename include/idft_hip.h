@@ -74,6 +74,19 @@ int idft_autotune_apply(idft_ctx* ctx, int32_t ncols, int32_t precision, int32_t
  * drives the contract-2 entry points without a host copy.  No counterpart
  * in the reference (its multivectors are the solver's own objects). */
 int idft_mv_wrap(idft_ctx* ctx, double* device_ptr, int32_t ncols, idft_mv** out);
+/* vmult (as idft_adj_vmult: Ax = H~ x - eps_i x_i, constrained rows zero)
+ * without any device allocation: the cell-product scratch is a fixed device
+ * array of the library (64 MiB per GPU, held until the process ends) instead
+ * of a buffer allocated on first use.  No counterpart in the reference; it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc), typically
+ * with Ax and x from idft_mv_wrap.  Requires cells * (p+1)^3 * block <=
+ * 8,388,608 (IDFT_EINVAL otherwise) and an operator that needs nothing built
+ * on first use: the sum-factorised cell product, or stored cell matrices
+ * already built (IDFT_ESTATE otherwise).  Every call on one GPU uses that
+ * GPU's one array, so calls that overlap on a GPU (other threads or
+ * contexts) race; the array's contents are fully rewritten by each call. */
+int idft_adj_vmult_noalloc(idft_ctx* ctx, idft_mv* Ax, idft_mv* x, int32_t block);
 /* Device pointer of a multivector handle (owned or wrapped). */
 double* idft_mv_device_ptr(const idft_mv* mv);
 

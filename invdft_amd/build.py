@@ -8,7 +8,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_lib")
 BUILD_DIR = os.path.join(HERE, "_lib", "obj")
-SOURCES = ["kernels.hip", "fe.cpp", "context.cpp", "bfgs.cpp", "vxc_io.cpp"]
+SOURCES = ["kernels.hip", "noalloc.hip", "fe.cpp", "context.cpp", "bfgs.cpp", "vxc_io.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]

@@ -200,6 +200,11 @@ int launch_cell_apply(const Geo& g, const CellOp<T>& op, const T* Sg, const TV* 
                       T* Yc, int B, const ApplyEpilogue<TV>& ep, const GemmShape& shape,
                       hipStream_t s);
 int64_t cell_scratch_elems(const Geo& g, int B);
+// Capacity (FP64 elements, 64 MiB) of the fixed device scratch of the
+// allocation-free operator entry point, and its device address on the current
+// device (noalloc.hip; looked up once per device).
+constexpr int64_t kNoallocScratchElems = int64_t(8) << 20;
+double* noalloc_scratch();
 // Phase 1 alone (per-cell products into Yc), for profiling.
 template <typename T>
 void launch_cell_gemm(const Geo& g, const CellOp<T>& op, const T* S, const T* X,

@@ -732,6 +732,37 @@ struct idft_ctx {
     apply<double>(x, b, ep);
   }
 
+  // vmult with the cell-product scratch in the library's fixed device array
+  // (noalloc.hip) instead of Yc64: no device allocation.  The operator must
+  // need nothing built on first use (the sum-factorised kernel, or stored
+  // cell matrices already built).  Calls that overlap on one GPU share the
+  // array and race.
+  void vmult_noalloc(double* Ax, double* x, int b) {
+    if (!adj.ready) throw Error(IDFT_ESTATE, "vmult before updateInputPsi");
+    if (!haveH) throw Error(IDFT_ESTATE, "no Hamiltonian: call set_potential first");
+    if (shape64.kernel != CK_SUM_FACTOR && !storedH)
+      throw Error(IDFT_ESTATE,
+                  "the selected cell-product kernel reads stored cell matrices, which are not "
+                  "built (building them allocates)");
+    const int64_t need = cell_scratch_elems(g, b);
+    if (need > kNoallocScratchElems)
+      throw Error(IDFT_EINVAL, "cells*(p+1)^3*block = " + std::to_string(need) +
+                                   " exceeds the fixed scratch of " +
+                                   std::to_string(kNoallocScratchElems) + " doubles");
+    double* scr = noalloc_scratch();
+    zero_boundary(x, b);
+    ApplyEpilogue<double> ep;
+    ep.Y = Ax; ep.alpha = 1.0;
+    ep.X = x; ep.a = 1.0; ep.ca = adj.dnegEig.p;
+    CellOp<double> o;
+    o.H = H64.p;
+    o.W = W64.p;
+    o.kin_scale = 0.25 * t.h;
+    st.applies_fp64++;
+    st.apply_cols_fp64 += b;
+    launch_cell_apply<double, double>(g, o, S.p, S.p, x, scr, b, ep, shape64, s);
+  }
+
   std::vector<double> dot_quad_wise(const double* v1, const double* v2, int b) {
     return pair_quad_sum(v1, v2, b, jxw.p, 0);
   }
@@ -1231,6 +1262,12 @@ int idft_hip_init(void) {
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, 0) != hipSuccess) return IDFT_EHIP;
   if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return IDFT_EINVAL;
+  // device address of the allocation-free entry point's fixed scratch
+  try {
+    (void)noalloc_scratch();
+  } catch (const Error& e) {
+    return e.code;
+  }
   return IDFT_OK;
 }
 
@@ -1477,6 +1514,16 @@ int idft_adj_vmult(idft_ctx* c, idft_mv* Ax, idft_mv* x, int32_t block) {
     if (block != c->adj.B || x->ncols != block || Ax->ncols != block)
       throw Error(IDFT_EINVAL, "block size mismatch");
     c->vmult(Ax->p(), x->p(), block);
+    c->sync();
+  });
+}
+
+int idft_adj_vmult_noalloc(idft_ctx* c, idft_mv* Ax, idft_mv* x, int32_t block) {
+  if (!c || !Ax || !x) return IDFT_EINVAL;
+  return guarded(c, [&] {
+    if (block != c->adj.B || x->ncols != block || Ax->ncols != block)
+      throw Error(IDFT_EINVAL, "block size mismatch");
+    c->vmult_noalloc(Ax->p(), x->p(), block);
     c->sync();
   });
 }
