@@ -10,12 +10,18 @@ iteration is two getForceVector evaluations.  Inputs are synthetic (the
 target density is the FP64 ground state of the wells alone; v_xc starts at a
 seeded sum of 8 Gaussians).
 
-The mixed-precision run (FP32-MFMA cell operator, defect-corrected FP32
-Chebyshev filter, FP64 MINRES recurrence whose cell operator drops to FP32
-once the residual is near the tolerance, FP64 everything else) is timed over
-K iterations after W warm-up iterations, then the FP64 run on the same
-problem.  Every library call returns only after its work on
-the GPU stream has completed, so host wall-clock brackets are synchronised.
+The mixed-precision run (FP32 sum-factorised cell operator, defect-corrected
+FP32 Chebyshev filter, FP64 MINRES recurrence whose cell operator drops to
+FP32 once the residual is near the tolerance, FP64 everything else) is timed
+over K iterations after W warm-up iterations, then the FP64 run on the same
+problem.  Every library call returns only after its work on the GPU stream
+has completed, so host wall-clock brackets are synchronised.
+
+After the headline line's measurements, `detail.c3` times one iteration of
+the batched workload (40^3 cells, degree 4, 4.17 M nodes, 64 occupied + 16
+buffer states) per precision: its two getForceVector evaluations, at the
+starting potential and at one step along the negative force (the CP line
+search's first point), with the force error of mixed against FP64 at both.
 """
 import argparse
 import json
@@ -108,15 +114,10 @@ def run_bfgs(lib, prob, precision, warmup, steps, tol_kw):
                      gnorm=list(gnorm), per_eval=per_eval, last=st1, evals=evals, tuned=tuned)
 
 
-def cpu_baseline(prob, per_eval, evals_per_iter):
-    """Oracle timing sample: one FP64 H_e build and one 12-column apply on this
-    mesh, scaled by the GPU run's per-iteration operation counts."""
-    from oracle import oracle
-    from invdft_amd import abi
-    olib = oracle.load()
+def _oracle_samples(olib, prob):
+    """Oracle H_e build and one n_eig-column Loewdin apply on prob's mesh (s)."""
     ctx = prob.context(olib)
     s = ctx.sizes
-    nq = s.n_cells * s.quad_per_cell
     t0 = time.perf_counter()
     ctx.set_veff_quad(prob.pot_base_q)
     t_build = time.perf_counter() - t0
@@ -125,17 +126,135 @@ def cpu_baseline(prob, per_eval, evals_per_iter):
     t0 = time.perf_counter()
     ctx.apply_hamiltonian(x)
     t_apply = time.perf_counter() - t0
-    cols = per_eval["apply_cols_fp64"] + per_eval["apply_cols_fp32"]
-    # per evaluation: H_e build, stored R_e build (same loop shape), column applies
-    t_eval = 2.0 * t_build + cols * t_apply / B
+    ctx.close()
+    return t_build, t_apply
+
+
+def cpu_baseline(lib, prob, per_eval, evals_per_iter, tol_kw):
+    """The oracle (FP64 CPU restatement) on the host cores.  The headline
+    workload's evaluation is too long to time whole on the CPU (~5 min), so
+    its value is a model: the oracle's H_e build and one n_eig-column apply
+    on this mesh, timed, scaled by the GPU run's per-evaluation count of
+    column applications plus two cell-matrix builds.  The model is checked
+    where a whole evaluation is affordable: one full oracle getForceVector
+    on the 16^3-cell degree-4 adjoint workload, timed, against the same
+    model built from that run's own counts."""
+    from oracle import oracle
+    from invdft_amd import synthetic
+    olib = oracle.load()
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    del nq
+    t_build, t_apply = _oracle_samples(olib, prob)
+    B = prob.desc.n_eig
+    cols = per_eval["apply_cols_fp64"] + per_eval["apply_cols_fp32"]
+    t_eval = 2.0 * t_build + cols * t_apply / B
+    # model check on the small workload: one whole evaluation, timed
+    cfg1 = dict(synthetic.CONFIGS["adjoint_16c_p4"])
+    p1 = synthetic.make_problem(lib, seed=1, target_tol=1e-10, **cfg1)
+    c1 = p1.context(olib, **tol_kw)
+    t0 = time.perf_counter()
+    c1.get_force(p1.vxc0)
+    t_meas = time.perf_counter() - t0
+    st = c1.stats()
+    c1.close()
+    b1, a1 = _oracle_samples(olib, p1)
+    t_model1 = 2.0 * b1 + st["apply_cols_fp64"] * a1 / p1.desc.n_eig
+    log(f"  oracle C1 evaluation {t_meas:.2f} s measured, {t_model1:.2f} s modelled")
     return {"value": round(t_eval * evals_per_iter, 3), "unit": "s/iter", "cores": cores,
             "kind": "port",
             "sample": (f"FP64 oracle on this mesh: H_e build {t_build:.2f} s + one {B}-column "
                        f"Loewdin apply {t_apply:.2f} s, scaled to {cols:.0f} column-applies and "
                        f"2 cell-matrix builds per evaluation, {evals_per_iter} evaluations per "
-                       f"iteration")}
+                       f"iteration"),
+            "measured_check": {"workload": "adjoint_16c_p4", "s_per_eval_measured": round(t_meas, 3),
+                               "s_per_eval_model": round(t_model1, 3),
+                               "model_rel_error": round(t_model1 / t_meas - 1.0, 4),
+                               "column_applies": st["apply_cols_fp64"]}}
+
+
+def c3_leg(lib, cheb_tol):
+    """One BFGS iteration of the batched workload per precision: getForceVector
+    at the starting potential x0 and at x0 - g0 (the CP line search's first
+    point), each context autotuned first (untimed)."""
+    from invdft_amd import abi, synthetic
+    cfg = dict(synthetic.CONFIGS["batched_40c_p4"])
+    t0 = time.perf_counter()
+    prob = synthetic.make_problem(lib, seed=2, target_tol=1e-10, **cfg)
+    log(f"  C3 set-up: {time.perf_counter() - t0:.1f} s")
+    kw = dict(cheb_tol=cheb_tol, initial_tol_cheb=cheb_tol)
+    out, forces = {}, {}
+    B = prob.desc.n_eig
+    for name, prec in (("mixed", abi.MIXED), ("fp64", abi.FP64)):
+        ctx = prob.context(lib, precision=prec, **kw)
+        ctx.set_potential(prob.vxc0)
+        tuned = {"fp64": ctx.autotune_apply(B, abi.FP64)}
+        if prec == abi.MIXED:
+            tuned["fp32"] = ctx.autotune_apply(B, abi.MIXED)
+        x0 = prob.vxc0.copy()
+        t0 = time.perf_counter()
+        f0, l0 = ctx.get_force(x0.copy())
+        st0 = ctx.stats()
+        f1, l1 = ctx.get_force(x0 - f0)
+        dt = time.perf_counter() - t0
+        st1 = ctx.stats()
+        forces[name] = (f0, f1)
+        ms64, cell64 = ctx.profile_apply(B, abi.FP64, reps=10)
+        out[name] = {"s_per_iter": round(dt, 3), "loss": [l0, l1],
+                     "eigen_ms": [round(st0["t_eigen_ms"], 1), round(st1["t_eigen_ms"], 1)],
+                     "adjoint_ms": [round(st0["t_adjoint_ms"], 1), round(st1["t_adjoint_ms"], 1)],
+                     "minres_iters_max": [st0["minres_iters_max"], st1["minres_iters_max"]],
+                     "minres_column_steps": st1["minres_column_steps"],
+                     "minres_compactions": st1["minres_compactions"],
+                     "tile_shapes": tuned,
+                     "fp64_cell_product_ms": round(cell64, 4), "fp64_operator_ms": round(ms64, 4)}
+        log(f"  C3 {name}: {out[name]}")
+        ctx.close()
+    fm, ff = forces["mixed"], forces["fp64"]
+    err = [float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in zip(fm, ff)]
+    return {"workload": "batched_40c_p4", "nodes": int(prob.desc.n_cells * prob.desc.degree + 1) ** 3,
+            "orbitals": B, "evals_per_iter": 2, "mixed": out["mixed"], "fp64": out["fp64"],
+            "speedup_mixed_vs_fp64": round(out["fp64"]["s_per_iter"] / out["mixed"]["s_per_iter"], 3),
+            "force_rel_err_mixed_vs_fp64": err}
+
+
+def c4_large(lib, evals):
+    """The large gradient workload (32^3 cells, degree 5, 4.17 M nodes, 32
+    orbitals, child 65^3): `evals` getForceVector evaluations with the FP64
+    eigenvector block resident between the passes and with it checkpointed
+    (FP32 copy, recomputed by one ChFSI pass for the backward pass); time per
+    evaluation, the library's peak device memory and the force difference."""
+    import ctypes as C
+    from invdft_amd import abi, synthetic
+    fn = lib.idft_device_memory
+    fn.restype = C.c_int
+    fn.argtypes = [C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.c_int32]
+    cur, peak = C.c_int64(), C.c_int64()
+    cfg = dict(synthetic.CONFIGS["gradcheck_32c_p5"])
+    t0 = time.perf_counter()
+    prob = synthetic.make_problem(lib, seed=4, target_tol=1e-10, **cfg)
+    log(f"set-up: {time.perf_counter() - t0:.1f} s")
+    out, forces = {}, {}
+    for ck in (0, 1):
+        fn(C.byref(cur), C.byref(peak), 1)
+        base = cur.value
+        ctx = prob.context(lib, psi_checkpoint=ck, cheb_tol=1e-8, initial_tol_cheb=1e-8)
+        t0 = time.perf_counter()
+        fs = []
+        for k in range(evals):
+            f, L = ctx.get_force(prob.vxc0 * (1.0 + 0.02 * k))
+            fs.append(f)
+        dt = (time.perf_counter() - t0) / evals
+        fn(C.byref(cur), C.byref(peak), 0)
+        st = ctx.stats()
+        out["checkpointed" if ck else "resident"] = {
+            "s_per_eval": round(dt, 3), "peak_device_GiB": round((peak.value - base) / 2**30, 3),
+            "psi_recomputes": st["psi_recomputes"], "t_eigen_ms": round(st["t_eigen_ms"], 1),
+            "t_adjoint_ms": round(st["t_adjoint_ms"], 1)}
+        forces[ck] = fs
+        log(f"  psi_checkpoint={ck}: {out['checkpointed' if ck else 'resident']}")
+        ctx.close()
+    err = [float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in zip(forces[1], forces[0])]
+    print(json.dumps({"workload": "gradcheck_32c_p5", "nodes": 161 ** 3, "orbitals": cfg["n_eig"],
+                      "evals": evals, **out, "force_rel_diff": err}), flush=True)
 
 
 def kernels_only(lib, cfg, shape, reps):
@@ -264,6 +383,9 @@ def main():
     ap.add_argument("--cheb-tol", type=float, default=1e-8)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--c3", type=int, default=1,
+                    help="1: time one iteration of the batched 40^3 workload per precision "
+                         "after the headline (detail.c3); 0: skip")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the operator kernels at --shape (for counter runs)")
     ap.add_argument("--adjoint-only", action="store_true",
@@ -291,6 +413,9 @@ def main():
     from invdft_amd import abi, synthetic
 
     lib = invdft_amd.load()
+    if args.config == "gradcheck_32c_p5":
+        c4_large(lib, args.steps)
+        return
     if args.config == "uneven_batch":
         uneven_batch(lib, args.batch_count, args.batch_steps, args.workers)
         return
@@ -399,7 +524,12 @@ def main():
     cpu = None
     if not args.no_cpu:
         log("CPU baseline sample (oracle)")
-        cpu = cpu_baseline(prob, mixed["per_eval"] if fp64 is None else fp64["per_eval"], 2)
+        cpu = cpu_baseline(lib, prob, mixed["per_eval"] if fp64 is None else fp64["per_eval"], 2,
+                           tol_kw)
+    c3 = None
+    if args.c3:
+        log("batched workload (C3), one iteration per precision")
+        c3 = c3_leg(lib, args.cheb_tol)
 
     out = {
         "metric": METRIC,
@@ -424,7 +554,7 @@ def main():
         "cpu_baseline": cpu,
         "detail": {"mixed_per_eval": mixed["per_eval"], "fp64_per_eval": fp64["per_eval"] if fp64 else None,
                    "tile_shapes": {"mixed": mixed["tuned"], "fp64": fp64["tuned"] if fp64 else None},
-                   "mixed_loss": mixed["loss"], **extra},
+                   "mixed_loss": mixed["loss"], "c3": c3, **extra},
     }
     print(json.dumps(out), flush=True)
 

@@ -135,6 +135,10 @@ typedef struct idft_stats {
   int32_t adjoint_blocks;     /* blocks of the last backward pass after the
                                  degeneracy-preserving shrink (ISF:930-955)   */
   int32_t psi_recomputes;     /* checkpoint recomputations (cumulative)       */
+  int32_t minres_compactions; /* HIP: repacks of the MINRES block to its open
+                                 columns (cumulative)                         */
+  int64_t minres_column_steps; /* HIP: MINRES steps x columns streamed
+                                  (cumulative)                                */
 } idft_stats;
 
 typedef struct idft_ctx idft_ctx;

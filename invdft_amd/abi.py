@@ -54,7 +54,8 @@ class Stats(C.Structure):
                 ("n_evaluations", C.c_int64), ("spectrum_upper_bound", C.c_double),
                 ("applies_fp64", C.c_int64), ("applies_fp32", C.c_int64),
                 ("apply_cols_fp64", C.c_int64), ("apply_cols_fp32", C.c_int64),
-                ("adjoint_blocks", C.c_int32), ("psi_recomputes", C.c_int32)]
+                ("adjoint_blocks", C.c_int32), ("psi_recomputes", C.c_int32),
+                ("minres_compactions", C.c_int32), ("minres_column_steps", C.c_int64)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}

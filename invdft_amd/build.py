@@ -8,7 +8,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_lib")
 BUILD_DIR = os.path.join(HERE, "_lib", "obj")
-SOURCES = ["kernels.hip", "noalloc.hip", "fe.cpp", "context.cpp", "bfgs.cpp", "vxc_io.cpp"]
+SOURCES = ["kernels.hip", "noalloc.hip", "setup.hip", "fe.cpp", "context.cpp", "bfgs.cpp",
+           "vxc_io.cpp", "ao.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
@@ -45,7 +46,7 @@ def build(verbose=True, incremental=False):
         objs = list(ex.map(lambda f: _compile(f, incremental), SOURCES))
     lib = os.path.join(OUT_DIR, "libidft.so")
     tmp = lib + ".tmp"
-    cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", *objs, "-o", tmp]
+    cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", *objs, "-lrocblas", "-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")

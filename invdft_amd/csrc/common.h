@@ -200,6 +200,9 @@ int launch_cell_apply(const Geo& g, const CellOp<T>& op, const T* Sg, const TV* 
                       T* Yc, int B, const ApplyEpilogue<TV>& ep, const GemmShape& shape,
                       hipStream_t s);
 int64_t cell_scratch_elems(const Geo& g, int B);
+// The FP64 even-odd coefficients of the sum-factorised kernel into constant
+// memory of the current device (once per device; every context calls it).
+void upload_sf_constants();
 // Capacity (FP64 elements, 64 MiB) of the fixed device scratch of the
 // allocation-free operator entry point, and its device address on the current
 // device (noalloc.hip; looked up once per device).
@@ -253,6 +256,11 @@ void launch_pair_quad(const Geo& g, const DevTables& t, const double* A, const d
                       hipStream_t s);
 void launch_copy_cols(int64_t N, int Bs, int off, int Bd, const double* src, double* dst,
                       hipStream_t s);
+// dst[n][k] = src[n][idx[k]], k < Bd (gather) / dst[n][idx[k]] = src[n][k], k < Bs (scatter)
+void launch_gather_cols(int64_t N, int Bs, const double* src, int Bd, const int* idx, double* dst,
+                        hipStream_t s);
+void launch_scatter_cols(int64_t N, int Bs, const double* src, int Bd, const int* idx, double* dst,
+                         hipStream_t s);
 // out[q] = u[q] * jxw[q % nq]
 void launch_mul_q(int64_t nQ, int nq, const double* u, const double* jxw, double* out,
                   hipStream_t s);

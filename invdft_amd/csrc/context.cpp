@@ -9,11 +9,13 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <set>
 #include <type_traits>
 
+#include "../../include/idft_setup.h"
 #include "common.h"
-#include "sf_tables.h"
+#include "setup.h"
 #include "sf_tables.h"
 
 using namespace idft;
@@ -84,6 +86,8 @@ struct idft_ctx {
   DevBuf<double> X, X2, HX, Xtmp, T2buf, rref, xdelta, psiAll, force;
   DevBuf<float> F0, F1, F2, Rf;
   DevBuf<double> ydev;
+  DevBuf<float> Xck;    // FP32 checkpoint of X between passes (psi_checkpoint)
+  bool checkpointed = false;  // X released, Xck holds it
   bool haveHX = false;  // HX == H~ X for the current X and Hamiltonian
   DevBuf<double> partial, small, small2, occD;
   std::vector<double> eig, occ, resid;
@@ -102,8 +106,8 @@ struct idft_ctx {
   idft_stats st{};
   AdjointState adj;
   MinresWork<double> mw64;
-  DevBuf<double> mst, mx;
-  DevBuf<int> mflags;
+  DevBuf<double> mst, mx, mtmp, mnegc;
+  DevBuf<int> mflags, msel;
   idft_mv mvPsi, mvX;
 
   void sync() { IDFT_HIP(hipStreamSynchronize(s)); }
@@ -815,87 +819,161 @@ struct idft_ctx {
   // its tolerance, the FP32 cell operator after that -- the matvec accuracy
   // a Krylov method needs falls as its residual falls (inexact-Krylov
   // relaxation), so the late steps tolerate FP32 products.
+  //
+  // Only open columns are streamed: the recurrence runs on a compact block of
+  // the columns not yet converged (`orig` maps its columns to the caller's),
+  // repacked at the host convergence checks once enough columns have
+  // finished, and a column whose right-hand side is already below its
+  // tolerance (an unoccupied state: f = 0 makes b~ = 0) never enters it.
+  // Each column's recurrence is independent of the others', so the iterates
+  // are those of the full block.
   std::vector<int> minres_core(const double* rhs, double* x, int b, const std::vector<double>& tol,
                                int maxit, double relax) {
     MinresWork<double>& W = mw64;
     W.ensure(NB(b));
+    mx.resize(NB(b));
+    mtmp.resize(NB(b));
     mst.resize((size_t)MS_COUNT * b);
     mflags.resize(b);
     mst.upload(tol.data(), b, s);  // slot 0 temporarily
     IDFT_HIP(hipMemcpyAsync(mst.p + (size_t)MS_TOL * b, mst.p, b * sizeof(double),
                             hipMemcpyDeviceToDevice, s));
     const int64_t N = t.N;
-    launch_lincomb<double>(N, b, W.r1.p, 0.0, rhs, 1.0, nullptr, nullptr, 0.0, nullptr, nullptr, s);
-    launch_lincomb<double>(N, b, W.r2.p, 0.0, rhs, 1.0, nullptr, nullptr, 0.0, nullptr, nullptr, s);
-    launch_lincomb<double>(N, b, W.w1.p, 0.0, nullptr, 0.0, nullptr, nullptr, 0.0, nullptr, nullptr, s);
-    launch_lincomb<double>(N, b, W.w.p, 0.0, nullptr, 0.0, nullptr, nullptr, 0.0, nullptr, nullptr, s);
+    double *r1 = W.r1.p, *r2 = W.r2.p, *w1 = W.w1.p, *w = W.w.p, *xc = mx.p;
+    launch_lincomb<double>(N, b, r1, 0.0, rhs, 1.0, nullptr, nullptr, 0.0, nullptr, nullptr, s);
+    launch_lincomb<double>(N, b, r2, 0.0, rhs, 1.0, nullptr, nullptr, 0.0, nullptr, nullptr, s);
+    launch_lincomb<double>(N, b, w1, 0.0, nullptr, 0.0, nullptr, nullptr, 0.0, nullptr, nullptr, s);
+    launch_lincomb<double>(N, b, w, 0.0, nullptr, 0.0, nullptr, nullptr, 0.0, nullptr, nullptr, s);
+    launch_lincomb<double>(N, b, xc, 0.0, nullptr, 0.0, nullptr, nullptr, 0.0, nullptr, nullptr, s);
     const bool canRelax = relax > 0.0 && mixed;
     // stored cell matrices (if a stored-H kernel is selected) built before
     // any step is captured into a graph
     (void)op64();
     if (canRelax) (void)op32();
-    int nblk = std::max(1024, gather_blocks(g, b, shape64.gather_rows, 8));
-    if (canRelax) nblk = std::max(nblk, gather_blocks(g, b, shape32.gather_rows, 8));
+    // dot partials of every block width the compaction may reach
+    int nblk = 1024;
+    for (int bw = 1; bw <= b; ++bw) {
+      nblk = std::max(nblk, gather_blocks(g, bw, shape64.gather_rows, 8));
+      if (canRelax) nblk = std::max(nblk, gather_blocks(g, bw, shape32.gather_rows, 8));
+    }
     partial.resize((size_t)nblk * b);
-    int nb = launch_coldot_partial<double>(N, b, W.r1.p, W.r1.p, Djac.p, partial.p, s);
+    int nb = launch_coldot_partial<double>(N, b, r1, r1, Djac.p, partial.p, s);
     launch_minres_start(b, nb, partial.p, adj.dnegEig.p, mst.p, s);
+    // the compact block: bc columns, orig[k] the caller's column of column k
+    int bc = b;
+    std::vector<int> orig(b);
+    for (int j = 0; j < b; ++j) orig[j] = j;
+    std::vector<double> negc(b);
+    for (int j = 0; j < b; ++j) negc[j] = -adj.eig[j];
+    mnegc.upload(negc.data(), b, s);
+    std::vector<int> its(b, 0);
     const int check = std::max(1, d.check_every);
     std::vector<int> flags(b);
     std::vector<double> phibar(b);
     bool f32op = false;
-    // r2 = r_k (newest Lanczos residual), r1 = r_{k-1}; the step's Lanczos
-    // vector v = cv D r2 is formed on the fly (not stored)
-    double *r1 = W.r1.p, *r2 = W.r2.p, *w1 = W.w1.p, *w = W.w.p;
     // one MINRES step: 5 launches, all arguments device-resident (scalars in
     // mst, done flags per column), so a run of steps can be replayed as a
     // graph.  The two buffer swaps return to the start after two steps.
-    // N*B streams per step: 1 (cell product) + 3 (gather) + 3 (Lanczos) + 6
-    // (update).
+    // N*bc streams per step: 1 (cell product) + 3 (gather) + 3 (Lanczos) + 6
+    // (update).  r2 = r_k (newest Lanczos residual), r1 = r_{k-1}; the step's
+    // Lanczos vector v = cv D r2 is formed on the fly (not stored).
     auto step = [&] {
       // y <- H~ v - eps v + cr1 r1, v = cv D r2;  partial dots of v.y  (one apply)
       ApplyEpilogue<double> ep;
       ep.Y = W.y.p; ep.alpha = 1.0;
-      ep.X = r2; ep.a = 1.0; ep.ca = mst.p + (size_t)MS_CVN * b;
-      ep.Z = r1; ep.b = 1.0; ep.cb = mst.p + (size_t)MS_CR1 * b;
+      ep.X = r2; ep.a = 1.0; ep.ca = mst.p + (size_t)MS_CVN * bc;
+      ep.Z = r1; ep.b = 1.0; ep.cb = mst.p + (size_t)MS_CR1 * bc;
       ep.xn = Djac.p;
-      ep.xcol = mst.p + (size_t)MS_CV * b;
+      ep.xcol = mst.p + (size_t)MS_CV * bc;
       ep.dot_partial = partial.p;
-      int nbk = f32op ? apply_f32op(r2, b, ep) : apply<double>(r2, b, ep);
-      launch_minres_alpha(b, nbk, partial.p, mst.p, s);
-      nbk = launch_minres_lanczos<double>(N, b, W.y.p, r2, r1, Djac.p, mst.p, partial.p, s);
-      launch_minres_givens(b, nbk, partial.p, adj.dnegEig.p, mst.p, s);
-      launch_minres_update<double>(N, b, r2, Djac.p, w1, w, x, mst.p, s);
+      int nbk = f32op ? apply_f32op(r2, bc, ep) : apply<double>(r2, bc, ep);
+      launch_minres_alpha(bc, nbk, partial.p, mst.p, s);
+      nbk = launch_minres_lanczos<double>(N, bc, W.y.p, r2, r1, Djac.p, mst.p, partial.p, s);
+      launch_minres_givens(bc, nbk, partial.p, mnegc.p, mst.p, s);
+      launch_minres_update<double>(N, bc, r2, Djac.p, w1, w, xc, mst.p, s);
       std::swap(r1, r2);
       std::swap(w1, w);
     };
     // check_every steps between host convergence checks, captured once per
-    // solve and operator precision as a HIP graph and replayed (kernels only:
-    // no memset or copy nodes); the apply scratch is sized before capture.
+    // compact block and operator precision as a HIP graph and replayed
+    // (kernels only: no memset or copy nodes); the apply scratch is sized
+    // before capture.
     const bool useGraph = d.minres_graph != 0 && check % 2 == 0;
     Yc64.resize(cell_scratch_elems(g, b));
     if (canRelax) Yc32.resize(cell_scratch_elems(g, b));
     hipGraphExec_t execs[2] = {nullptr, nullptr};
     struct ExecGuard {
       hipGraphExec_t* e;
-      ~ExecGuard() {
+      void drop() {
         for (int i = 0; i < 2; ++i)
-          if (e[i]) (void)hipGraphExecDestroy(e[i]);
+          if (e[i]) { (void)hipGraphExecDestroy(e[i]); e[i] = nullptr; }
       }
+      ~ExecGuard() { drop(); }
     } guard{execs};
-    for (int k = 0; k < maxit;) {
-      launch_copy_done(b, mst.p, mflags.p, s);
-      IDFT_HIP(hipMemcpyAsync(flags.data(), mflags.p, b * sizeof(int), hipMemcpyDeviceToHost, s));
+    std::vector<double> hst;
+    // the one buffer not holding a block vector (the buffers stay owned by
+    // their DevBufs; the block vectors are views that move between them)
+    double* spare = mtmp.p;
+    // Repack the compact block to the columns `keep` (positions in it): the
+    // caller's x receives every current column first, then the vectors, the
+    // per-column state and the shifts are gathered.
+    auto compact = [&](const std::vector<int>& keep) {
+      const int bn = (int)keep.size();
+      msel.upload(orig.data(), bc, s);
+      launch_scatter_cols(N, bc, xc, b, msel.p, x, s);
+      hst.resize((size_t)MS_COUNT * bc);
+      IDFT_HIP(hipMemcpyAsync(hst.data(), mst.p, hst.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+      sync();
+      for (int k = 0; k < bc; ++k) its[orig[k]] = (int)hst[(size_t)MS_IT * bc + k];
+      if (bn == 0) { bc = 0; return; }
+      msel.upload(keep.data(), bn, s);
+      for (double** buf : {&r1, &r2, &w1, &w, &xc}) {
+        launch_gather_cols(N, bc, *buf, bn, msel.p, spare, s);
+        std::swap(*buf, spare);
+      }
+      std::vector<double> nst((size_t)MS_COUNT * bn);
+      std::vector<int> norig(bn);
+      std::vector<double> nneg(bn);
+      for (int i = 0; i < bn; ++i) {
+        for (int k = 0; k < MS_COUNT; ++k) nst[(size_t)k * bn + i] = hst[(size_t)k * bc + keep[i]];
+        norig[i] = orig[keep[i]];
+        nneg[i] = negc[keep[i]];
+      }
+      mst.upload(nst.data(), nst.size(), s);
+      mnegc.upload(nneg.data(), bn, s);
+      orig = norig;
+      negc = nneg;
+      bc = bn;
+      guard.drop();
+      sync();
+    };
+    int k = 0;
+    for (;;) {
+      launch_copy_done(bc, mst.p, mflags.p, s);
+      IDFT_HIP(hipMemcpyAsync(flags.data(), mflags.p, bc * sizeof(int), hipMemcpyDeviceToHost, s));
       if (canRelax && !f32op)
-        IDFT_HIP(hipMemcpyAsync(phibar.data(), mst.p + (size_t)MS_PHIBAR * b, b * sizeof(double),
+        IDFT_HIP(hipMemcpyAsync(phibar.data(), mst.p + (size_t)MS_PHIBAR * bc, bc * sizeof(double),
                                 hipMemcpyDeviceToHost, s));
       sync();
-      bool all = true;
-      for (int f : flags) all = all && f;
-      if (all) break;
+      std::vector<int> open;
+      for (int j = 0; j < bc; ++j)
+        if (!flags[j]) open.push_back(j);
+      if (open.empty() || k >= maxit) break;
+      if ((int)open.size() < bc && bc - (int)open.size() >= std::max(1, bc / 16)) {
+        compact(open);
+        st.minres_compactions++;
+        if (verbose)
+          std::fprintf(stderr, "[idft] minres: step %d, block compacted to %d columns\n", k, bc);
+        // the compacted flags and estimates are the open columns' values
+        std::vector<double> pb(bc);
+        for (int i = 0; i < bc; ++i) pb[i] = phibar[open[i]];
+        phibar = pb;
+        flags.assign(bc, 0);
+      }
       if (canRelax && !f32op) {
         bool ready = true;
-        for (int j = 0; j < b; ++j)
-          if (!flags[j] && !(phibar[j] <= relax * tol[j])) ready = false;
+        for (int j = 0; j < bc; ++j)
+          if (!flags[j] && !(phibar[j] <= relax * tol[orig[j]])) ready = false;
         if (ready) {
           f32op = true;
           if (verbose)
@@ -903,6 +981,7 @@ struct idft_ctx {
         }
       }
       const int n = std::min(check, maxit - k);
+      st.minres_column_steps += (int64_t)n * bc;
       if (useGraph && n == check) {
         hipGraphExec_t& exec = execs[f32op ? 1 : 0];
         if (!exec) {
@@ -922,10 +1001,10 @@ struct idft_ctx {
         } else if (f32op) {
           // the capture counted its n steps; replays count theirs here
           st.applies_fp32 += n;
-          st.apply_cols_fp32 += (int64_t)n * b;
+          st.apply_cols_fp32 += (int64_t)n * bc;
         } else {
           st.applies_fp64 += n;
-          st.apply_cols_fp64 += (int64_t)n * b;
+          st.apply_cols_fp64 += (int64_t)n * bc;
         }
         IDFT_HIP(hipGraphLaunch(exec, s));
       } else {
@@ -933,13 +1012,9 @@ struct idft_ctx {
       }
       k += n;
     }
-    std::vector<double> its(b);
-    IDFT_HIP(hipMemcpyAsync(its.data(), mst.p + (size_t)MS_IT * b, b * sizeof(double),
-                            hipMemcpyDeviceToHost, s));
-    sync();
-    std::vector<int> out(b);
-    for (int j = 0; j < b; ++j) out[j] = (int)its[j];
-    return out;
+    // the caller's x and the iteration counts of the columns still in the block
+    if (bc > 0) compact({});
+    return its;
   }
 
   void minres(idft_mv* x, double tol, int maxit, int distribute, int32_t* iters) {
@@ -1021,6 +1096,249 @@ struct idft_ctx {
     sync();
   }
 
+  // ------------------------------------- set-up stage solvers (idft_setup.h)
+  // One column: y = Sout (A (Sin x)) with the sum-factorised cell operator o
+  // (quadrature weights o.W, kinetic scale o.kin_scale).
+  void cell_apply1(const CellOp<double>& o, const double* Sin, const double* Sout, const double* x,
+                   double* y) {
+    GemmShape sh;
+    sh.kernel = CK_SUM_FACTOR;
+    sh.vectors_per_tile = 2;
+    sh.cells_per_wg = 1;
+    Yc64.resize(cell_scratch_elems(g, 1));
+    ApplyEpilogue<double> ep;
+    ep.Y = y;
+    ep.alpha = 1.0;
+    launch_cell_apply<double, double>(g, o, Sin, Sout, x, Yc64.p, 1, ep, sh, s);
+  }
+  double dot1(const double* a, const double* b, const double* w = nullptr) {
+    return coldot(a, b, 1, w)[0];
+  }
+  // Preconditioned CG from x = 0 on one column: A(in, out) applies the
+  // operator, D is the diagonal preconditioner; stops at ||r||_D <= tol
+  // ||b||_D.  Returns the iterations; relres gets the final ratio.
+  template <typename Op>
+  int pcg1(Op&& A, const double* b, const double* D, double tol, int maxit, double* x,
+           double* relres) {
+    const int64_t N = t.N;
+    DevBuf<double> r(N), z(N), p(N), q(N);
+    launch_lincomb<double>(N, 1, x, 0.0, nullptr, 0.0, nullptr, nullptr, 0.0, nullptr, nullptr, s);
+    launch_lincomb<double>(N, 1, r.p, 0.0, b, 1.0, nullptr, nullptr, 0.0, nullptr, nullptr, s);
+    launch_lincomb<double>(N, 1, z.p, 0.0, r.p, 1.0, nullptr, nullptr, 0.0, nullptr, D, s);
+    launch_lincomb<double>(N, 1, p.p, 0.0, z.p, 1.0, nullptr, nullptr, 0.0, nullptr, nullptr, s);
+    double rz = dot1(r.p, z.p);
+    const double bn = std::sqrt(std::max(rz, 0.0));
+    double rn = bn;
+    int it = 0;
+    while (it < maxit && rn > tol * bn && bn > 0.0) {
+      A(p.p, q.p);
+      const double alpha = rz / dot1(p.p, q.p);
+      launch_lincomb<double>(N, 1, x, 1.0, p.p, alpha, nullptr, nullptr, 0.0, nullptr, nullptr, s);
+      launch_lincomb<double>(N, 1, r.p, 1.0, q.p, -alpha, nullptr, nullptr, 0.0, nullptr, nullptr, s);
+      launch_lincomb<double>(N, 1, z.p, 0.0, r.p, 1.0, nullptr, nullptr, 0.0, nullptr, D, s);
+      const double rz2 = dot1(r.p, z.p);
+      rn = std::sqrt(std::max(rz2, 0.0));
+      const double beta = rz2 / rz;
+      rz = rz2;
+      launch_lincomb<double>(N, 1, p.p, beta, z.p, 1.0, nullptr, nullptr, 0.0, nullptr, nullptr, s);
+      ++it;
+    }
+    if (relres) *relres = bn > 0.0 ? rn / bn : 0.0;
+    return it;
+  }
+  // f[node] = sum over cells and q of N_a(q) w[e][q] (w already holds JxW):
+  // the quadrature operator N^T diag(w) N applied to the all-ones field
+  // (N 1 = 1 by the partition of unity)
+  void integrate_parent(const double* w, double* f) {
+    DevBuf<double> ones(t.N);
+    fill_ones(ones.p, t.N);
+    launch_lincomb<double>(t.N, 1, f, 0.0, nullptr, 0.0, nullptr, nullptr, 0.0, nullptr, nullptr, s);
+    for (int colour = 0; colour < 8; ++colour) launch_quad_operator(g, dt, ones.p, 1, w, f, colour, s);
+  }
+  void fill_ones(double* v, int64_t n) {
+    std::vector<double> h((size_t)n, 1.0);
+    IDFT_HIP(hipMemcpyAsync(v, h.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
+    sync();
+  }
+  // Poisson: -lap phi = 4 pi c with phi = g on the boundary, as
+  //   (1/2 K~) phi~ = S (f/2 - (1/2 K) g),  phi = S phi~ + g,
+  // K~ = S K S with S = M^{-1/2} on free nodes (0 on the boundary), CG with
+  // the Jacobi diagonal D = 2 M / K_ii.
+  void poisson(const double* cq_host, const double* bc_host, double tol, int maxit,
+               double* phi_q_host, double* phi_nodal_host, idft_poisson_stats* pst) {
+    const int64_t N = t.N;
+    // charge moments about the origin and the boundary values
+    std::vector<double> gx, gq(nQ);
+    double Q = 0.0, dip[3] = {0, 0, 0};
+    {
+      std::vector<double> xyz((size_t)nQ * 3);
+      idft_geometry_quad_points(&d, xyz.data());
+      std::vector<double> jw(t.nq);
+      const double hh = 0.5 * t.h, J = hh * hh * hh;
+      for (int q = 0; q < t.nq; ++q)
+        jw[q] = J * t.gq_w[q % t.q1] * t.gq_w[(q / t.q1) % t.q1] * t.gq_w[q / (t.q1 * t.q1)];
+      for (int64_t i = 0; i < nQ; ++i) {
+        const double w = cq_host[i] * jw[i % t.nq];
+        Q += w;
+        for (int k = 0; k < 3; ++k) dip[k] += w * xyz[3 * i + k];
+      }
+    }
+    std::vector<double> gb((size_t)N, 0.0), bm((size_t)N, 0.0);
+    {
+      std::vector<double> nx((size_t)N * 3);
+      idft_geometry_node_points(&d, nx.data());
+      for (int64_t n = 0; n < N; ++n) {
+        const int64_t i = n % t.N1, j = (n / t.N1) % t.N1, k = n / ((int64_t)t.N1 * t.N1);
+        if (!(i == 0 || j == 0 || k == 0 || i == t.N1 - 1 || j == t.N1 - 1 || k == t.N1 - 1)) continue;
+        bm[n] = 1.0;
+        if (bc_host) { gb[n] = bc_host[n]; continue; }
+        const double* x = &nx[3 * n];
+        const double r = std::sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+        gb[n] = Q / r + (dip[0] * x[0] + dip[1] * x[1] + dip[2] * x[2]) / (r * r * r);
+      }
+    }
+    DevBuf<double> gd, bmask, w4, f, kg, b(N), xt(N), zeros(nQ);
+    gd.upload(gb.data(), N, s);
+    bmask.upload(bm.data(), N, s);
+    std::vector<double> c4((size_t)nQ);
+    for (int64_t i = 0; i < nQ; ++i) c4[i] = 4.0 * M_PI * cq_host[i];
+    DevBuf<double> c4d;
+    c4d.upload(c4.data(), nQ, s);
+    w4.resize(nQ);
+    launch_mul_q(nQ, t.nq, c4d.p, jxw.p, w4.p, s);
+    f.resize(N);
+    integrate_parent(w4.p, f.p);
+    launch_lincomb<double>(nQ, 1, zeros.p, 0.0, nullptr, 0.0, nullptr, nullptr, 0.0, nullptr,
+                           nullptr, s);
+    CellOp<double> kin;
+    kin.W = zeros.p;
+    kin.kin_scale = 0.25 * t.h;
+    kg.resize(N);
+    cell_apply1(kin, bmask.p, S.p, gd.p, kg.p);  // S (1/2 K) g
+    launch_lincomb<double>(N, 1, b.p, 0.0, f.p, 0.5, nullptr, nullptr, 0.0, nullptr, S.p, s);
+    launch_lincomb<double>(N, 1, b.p, 1.0, kg.p, -1.0, nullptr, nullptr, 0.0, nullptr, nullptr, s);
+    double rr = 0.0;
+    const int it = pcg1([&](const double* in, double* out) { cell_apply1(kin, S.p, S.p, in, out); },
+                        b.p, Djac.p, tol, maxit, xt.p, &rr);
+    DevBuf<double> phi(N);
+    // S is zero on the boundary, so phi = S phi~ there is g alone; in the
+    // interior the boundary term g is zero
+    launch_lincomb<double>(N, 1, phi.p, 0.0, xt.p, 1.0, nullptr, nullptr, 0.0, nullptr, S.p, s);
+    launch_lincomb<double>(N, 1, phi.p, 1.0, gd.p, 1.0, nullptr, nullptr, 0.0, nullptr, nullptr, s);
+    if (phi_nodal_host) phi.download(phi_nodal_host, N, s);
+    if (phi_q_host) {
+      DevBuf<double> pq(nQ);
+      launch_nodal_to_quad(g, dt.Lq.p, phi.p, pq.p, s);
+      pq.download(phi_q_host, nQ, s);
+    }
+    sync();
+    if (pst) {
+      pst->iterations = it;
+      pst->rel_residual = rr;
+      pst->charge = Q;
+      for (int k = 0; k < 3; ++k) pst->dipole[k] = dip[k];
+    }
+    if (it >= maxit && rr > tol) throw Error(IDFT_ENOCONV, "Poisson CG did not converge");
+  }
+  // L2 projection of quadrature values v onto the parent nodes with zero
+  // boundary: (S M S) x~ = S (int N v), x = S x~, M the consistent mass at the
+  // parent quadrature (the cell operator with weights JxW, no kinetic part).
+  void l2_project_parent(const double* vq_dev, double* x_dev) {
+    const int64_t N = t.N;
+    DevBuf<double> w(nQ), f(N), b(N), xt(N), ones(nQ), jw(nQ);
+    launch_mul_q(nQ, t.nq, vq_dev, jxw.p, w.p, s);
+    integrate_parent(w.p, f.p);
+    launch_lincomb<double>(N, 1, b.p, 0.0, f.p, 1.0, nullptr, nullptr, 0.0, nullptr, S.p, s);
+    fill_ones(ones.p, nQ);
+    launch_mul_q(nQ, t.nq, ones.p, jxw.p, jw.p, s);
+    CellOp<double> mass;
+    mass.W = jw.p;
+    mass.kin_scale = 0.0;
+    DevBuf<double> one(N);
+    fill_ones(one.p, N);
+    double rr = 0.0;
+    const int it = pcg1([&](const double* in, double* out) { cell_apply1(mass, S.p, S.p, in, out); },
+                        b.p, one.p, 1e-14, 500, xt.p, &rr);
+    if (it >= 500 && rr > 1e-14) throw Error(IDFT_ENOCONV, "L2 projection CG did not converge");
+    launch_lincomb<double>(N, 1, x_dev, 0.0, xt.p, 1.0, nullptr, nullptr, 0.0, nullptr, S.p, s);
+  }
+  // Parent nodal field -> child Q1 nodes: values at the child Gauss points
+  // times `scale`, integrated against the child basis, child consistent mass
+  // (the tensor product of 1D Q1 masses) inverted by line solves.
+  void parent_to_child(const double* x_dev, double scale, double* out_dev) {
+    DevBuf<double> ones(t.N);
+    fill_ones(ones.p, t.N);
+    launch_lincomb<double>(t.NV, 1, out_dev, 0.0, nullptr, 0.0, nullptr, nullptr, 0.0, nullptr,
+                           nullptr, s);
+    const double jc = scale * std::pow(0.25 * t.h, 3);
+    for (int colour = 0; colour < 8; ++colour)
+      launch_force(g, dt, x_dev, ones.p, 1, 0, jc, jxw.p, out_dev, colour, s);
+    std::vector<double> cp, inv;
+    const double hc = 0.5 * t.h;
+    child_mass_factors(t.NV1, hc, cp, inv);
+    DevBuf<double> cpd, invd;
+    cpd.upload(cp.data(), cp.size(), s);
+    invd.upload(inv.data(), inv.size(), s);
+    launch_child_mass_solve(t.NV1, hc, cpd.p, invd.p, out_dev, s);
+    sync();
+  }
+
+  // ------------------------------------------- eigenvector checkpointing
+  // With desc.psi_checkpoint the FP64 block X lives only while a pass needs
+  // it: between the forward and the backward pass (and between evaluations)
+  // it is kept as an FP32 copy, and the eigensolver's and the adjoint's work
+  // buffers are released, so the peak device memory is the larger of the two
+  // passes' working sets instead of their sum.
+  void release_eigen_work() {
+    X2.release(); HX.release(); Xtmp.release();
+    F0.release(); F1.release(); F2.release(); Rf.release();
+    haveHX = false;
+  }
+  void release_adjoint_work() {
+    mw64.r1.release(); mw64.r2.release(); mw64.y.release(); mw64.w1.release();
+    mw64.w.release(); mw64.rhs.release();
+    mx.release(); mtmp.release(); rref.release(); xdelta.release(); T2buf.release();
+    psiAll.release(); mvPsi.own.release(); mvX.own.release();
+    Yc64.release(); Yc32.release();
+  }
+  void checkpoint_x() {
+    Xck.resize(NB(B));
+    launch_convert_d2f((int64_t)NB(B), Xck.p, X.p, s);
+    sync();
+    X.release();
+    release_eigen_work();
+    checkpointed = true;
+  }
+  void restore_x() {
+    if (!checkpointed) return;
+    X.resize(NB(B));
+    launch_convert_f2d((int64_t)NB(B), X.p, Xck.p, s);
+    checkpointed = false;
+  }
+  // Runs f with X present, restored from the checkpoint for the call only.
+  template <typename F>
+  void with_x(F&& f) {
+    const bool was = checkpointed;
+    restore_x();
+    f();
+    if (was) {
+      sync();
+      X.release();
+      checkpointed = true;
+    }
+  }
+  // The FP64 block of the last forward pass, recomputed from its FP32
+  // checkpoint by one ChFSI pass (filter, orthonormalisation, Rayleigh-Ritz)
+  // on the same Hamiltonian: the filter takes the rounding of the copy out
+  // again.  The occupancies of the forward pass are kept.
+  void recompute_x() {
+    restore_x();
+    haveHX = false;
+    chfsi_pass();
+    st.psi_recomputes++;
+    release_eigen_work();
+  }
+
   // ------------------------------------------------------ getForceVector
   void forward(const double* pot, double* lossOut) {
     const double t0 = wall_ms();
@@ -1033,6 +1351,7 @@ struct idft_ctx {
     }
     if (vxcFreq > 0) potEval.assign(pot, pot + t.NV);
     set_potential(pot);
+    restore_x();  // warm start from the checkpoint of the previous evaluation
     solve_passes(tolCheb, d.max_cheb_passes, nullptr, nullptr);
     st.cheb_tol_used = tolCheb;
     sync();
@@ -1054,11 +1373,13 @@ struct idft_ctx {
     st.t_density_ms = wall_ms() - t1;
     if (lossOut) *lossOut = loss;
     haveForward = true;
+    if (d.psi_checkpoint) checkpoint_x();
   }
 
   void backward(double* forceOut) {
     if (!haveForward) throw Error(IDFT_ESTATE, "backward before forward");
     const double t0 = wall_ms();
+    if (checkpointed) recompute_x();
     physical_psi();
     force.resize(t.NV);
     launch_lincomb<double>(t.NV, 1, force.p, 0.0, nullptr, 0.0, nullptr, nullptr, 0.0, nullptr, nullptr, s);
@@ -1128,6 +1449,10 @@ struct idft_ctx {
     counter++;
     st.n_evaluations = counter;
     haveForward = false;
+    if (d.psi_checkpoint) {
+      release_adjoint_work();
+      checkpoint_x();
+    }
   }
 };
 
@@ -1171,6 +1496,7 @@ idft_ctx* create_ctx(const idft_desc* d, const double* pot_base_q, const double*
     c->dt.Wq.upload(t.Wq.data(), t.Wq.size(), s);
     c->dt.Wc.upload(t.Wc.data(), t.Wc.size(), s);
     c->check_sf_tables();
+    upload_sf_constants();
     // default cell product: sum-factorised (no stored H_e) for blocks of up
     // to 32 columns, where it is the faster kernel, if its cell scratch fits
     // 32-bit indexing; the row pass over stored tiles otherwise
@@ -1344,6 +1670,7 @@ int idft_eigensolve(idft_ctx* c, double tol, int32_t maxp, double* ev, double* m
   if (!c) return IDFT_EINVAL;
   return guarded(c, [&] {
     int p = 0;
+    c->restore_x();
     c->solve_passes(tol, maxp, &p, mr);
     if (passes) *passes = p;
     if (ev) std::copy(c->eig.begin(), c->eig.end(), ev);
@@ -1355,15 +1682,19 @@ int idft_get_eigenvectors(const idft_ctx* cc, double* psi) {
   if (!c || !psi) return IDFT_EINVAL;
   if (!c->haveX) return IDFT_ESTATE;
   return guarded(c, [&] {
-    c->physical_psi();
-    c->psiAll.download(psi, c->NB(c->B), c->s);
-    c->sync();
+    // between passes with psi_checkpoint: the FP32 checkpoint's values
+    c->with_x([&] {
+      c->physical_psi();
+      c->psiAll.download(psi, c->NB(c->B), c->s);
+      c->sync();
+    });
   });
 }
 
 int idft_set_eigenvectors(idft_ctx* c, const double* psi) {
   if (!c || !psi) return IDFT_EINVAL;
   return guarded(c, [&] {
+    c->checkpointed = false;
     c->X.upload(psi, c->NB(c->B), c->s);
     launch_lincomb<double>(c->t.N, c->B, c->X.p, 0.0, c->X.p, 1.0, nullptr, nullptr, 0.0, nullptr,
                            c->sqrtM.p, c->s);
@@ -1392,7 +1723,7 @@ int idft_get_density(idft_ctx* c, double* rho) {
   }
   return guarded(c, [&] {
     DevBuf<double> r(c->nQ);
-    c->density_into(r.p);
+    c->with_x([&] { c->density_into(r.p); });
     r.download(rho, c->nQ, c->s);
     c->sync();
   });
@@ -1683,6 +2014,97 @@ int idft_autotune_apply(idft_ctx* c, int32_t ncols, int32_t prec, int32_t reps,
     if (kernel) *kernel = best.kernel;
     if (gather_rows) *gather_rows = best.gather_rows;
     if (ms_best) *ms_best = bestMs;
+  });
+}
+
+// ------------------------------------------------ set-up stage (idft_setup.h)
+}  // extern "C"
+namespace {
+// A context that only carries the mesh of d (zero potential and density).
+std::unique_ptr<idft_ctx> mesh_ctx(const idft_desc* d) {
+  idft_desc m = *d;
+  m.n_eig = 2;
+  m.n_electrons = 2.0;
+  m.precision = IDFT_FP64;
+  if (!desc_valid(&m)) throw Error(IDFT_EINVAL, "invalid descriptor");
+  Tables t;
+  t.init(m);
+  const int64_t nQ = t.ncell * (int64_t)t.nq;
+  std::vector<double> zq((size_t)nQ, 0.0), oq((size_t)nQ, 1.0), on((size_t)t.N, 1.0);
+  return std::unique_ptr<idft_ctx>(create_ctx(&m, zq.data(), zq.data(), oq.data(), on.data()));
+}
+template <typename F>
+int guarded_free(F&& f) {
+  try {
+    f();
+    return IDFT_OK;
+  } catch (const Error& e) {
+    return e.code;
+  } catch (const std::exception&) {
+    return IDFT_EINVAL;
+  }
+}
+}  // namespace
+extern "C" {
+
+int idft_poisson_solve(const idft_desc* d, const double* charge_q, const double* boundary_nodal,
+                       double tol, int32_t max_iter, double* phi_q, double* phi_nodal,
+                       idft_poisson_stats* st) {
+  if (!d || !charge_q || !(tol > 0.0) || max_iter < 1) return IDFT_EINVAL;
+  return guarded_free([&] {
+    auto c = mesh_ctx(d);
+    c->poisson(charge_q, boundary_nodal, tol, max_iter, phi_q, phi_nodal, st);
+  });
+}
+
+int idft_l2_project_parent(const idft_desc* d, const double* v_q, double* x_nodal) {
+  if (!d || !v_q || !x_nodal) return IDFT_EINVAL;
+  return guarded_free([&] {
+    auto c = mesh_ctx(d);
+    DevBuf<double> vq, x(c->t.N);
+    vq.upload(v_q, c->nQ, c->s);
+    c->l2_project_parent(vq.p, x.p);
+    x.download(x_nodal, c->t.N, c->s);
+    c->sync();
+  });
+}
+
+int idft_parent_to_child(const idft_desc* d, const double* x_nodal, double scale, double* vxc_child) {
+  if (!d || !x_nodal || !vxc_child) return IDFT_EINVAL;
+  return guarded_free([&] {
+    auto c = mesh_ctx(d);
+    DevBuf<double> x, out(c->t.NV);
+    x.upload(x_nodal, c->t.N, c->s);
+    c->parent_to_child(x.p, scale, out.p);
+    out.download(vxc_child, c->t.NV, c->s);
+    c->sync();
+  });
+}
+
+int idft_initial_vxc(const idft_desc* d, const idft_guess_params* gp, const double* rho_spin_q,
+                     const double* sigma_q, const double* v_hartree_q, double* v_lda_q,
+                     double* vxc_child) {
+  if (!d || !gp || !rho_spin_q || (gp->lb94 && !sigma_q) ||
+      (gp->fermi_amaldi && (!v_hartree_q || !(gp->n_electrons > 0.0))))
+    return IDFT_EINVAL;
+  return guarded_free([&] {
+    auto c = mesh_ctx(d);
+    const int64_t nQ = c->nQ;
+    DevBuf<double> rho, sig, vh, v(nQ);
+    rho.upload(rho_spin_q, nQ, c->s);
+    if (gp->lb94) sig.upload(sigma_q, nQ, c->s);
+    if (gp->fermi_amaldi) vh.upload(v_hartree_q, nQ, c->s);
+    GuessArgs ga{gp->lb94, gp->fermi_amaldi, gp->tau_vx, gp->tau_fa, gp->factor_fa,
+                 gp->spin_factor, gp->n_electrons};
+    launch_guess(nQ, ga, rho.p, sig.p, vh.p, v.p, c->s);
+    if (v_lda_q) v.download(v_lda_q, nQ, c->s);
+    if (vxc_child) {
+      DevBuf<double> x(c->t.N), out(c->t.NV);
+      c->l2_project_parent(v.p, x.p);
+      c->parent_to_child(x.p, 1.0 - gp->factor_lda, out.p);
+      out.download(vxc_child, c->t.NV, c->s);
+    }
+    c->sync();
   });
 }
 

@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <type_traits>
 
 #include "common.h"
@@ -544,6 +545,7 @@ __device__ inline void pack_fma(Pack<T, VC>& acc, T c, const Pack<T, VC>& x) {
 // A[R-1-i][C-1-j] = A[i][j] (to rounding of the tables).
 template <int P, int K>
 struct SfTab {
+  static constexpr int kP = P, kK = K;
   static constexpr int N1 = P + 1, Q = P + 3;
   static constexpr int R = K == 0 ? Q : N1;
   static constexpr int C = K == 1 ? Q : N1;
@@ -577,6 +579,77 @@ __device__ constexpr T eo_mid() {
   constexpr double c = A::at(I, A::C / 2);
   return (T)c;
 }
+// FP64 coefficients from constant memory.  A double has no literal operand,
+// so an immediate FP64 coefficient costs two scalar moves per use; the FP64
+// kernels instead read the even-odd coefficients of all four operators from
+// one table in constant memory, which the compiler fetches with wide scalar
+// loads (4 doubles per load).  Per operator K (R x C) and row I < ceil(R/2):
+// the floor(C/2) even coefficients, the floor(C/2) odd ones, then the middle
+// one if C is odd.
+template <int P>
+struct EoTable {
+  static constexpr int R(int k) { return k == 0 ? P + 3 : P + 1; }
+  static constexpr int C(int k) { return k == 1 ? P + 3 : P + 1; }
+  static constexpr int stride(int k) { return 2 * (C(k) / 2) + C(k) % 2; }
+  static constexpr int base(int k) {
+    int b = 0;
+    for (int j = 0; j < k; ++j) b += ((R(j) + 1) / 2) * stride(j);
+    return b;
+  }
+  static constexpr int kSize = base(4);
+  double v[kSize];
+};
+template <int P>
+constexpr EoTable<P> make_eo_table() {
+  EoTable<P> t{};
+  for (int k = 0; k < 4; ++k) {
+    const int r = EoTable<P>::R(k), c = EoTable<P>::C(k), st = EoTable<P>::stride(k);
+    for (int i = 0; i < (r + 1) / 2; ++i) {
+      double* row = t.v + EoTable<P>::base(k) + i * st;
+      auto at = [&](int ii, int jj) {
+        return k == 0   ? SfC<P>::v.L[ii * (P + 1) + jj]
+               : k == 1 ? SfC<P>::v.L[jj * (P + 1) + ii]
+               : k == 2 ? SfC<P>::v.M[ii * (P + 1) + jj]
+                        : SfC<P>::v.D[ii * (P + 1) + jj];
+      };
+      for (int j = 0; j < c / 2; ++j) {
+        row[j] = 0.5 * (at(i, j) + at(i, c - 1 - j));
+        row[c / 2 + j] = 0.5 * (at(i, j) - at(i, c - 1 - j));
+      }
+      if (c % 2) row[2 * (c / 2)] = at(i, c / 2);
+    }
+  }
+  return t;
+}
+// Written from the host once per device (upload_sf_constants): with an
+// initialiser the compiler would fold the values back into immediates.
+__constant__ EoTable<2> kEo2;
+__constant__ EoTable<3> kEo3;
+__constant__ EoTable<4> kEo4;
+__constant__ EoTable<5> kEo5;
+template <int P> __device__ inline const EoTable<P>& eo_table();
+template <> __device__ inline const EoTable<2>& eo_table<2>() { return kEo2; }
+template <> __device__ inline const EoTable<3>& eo_table<3>() { return kEo3; }
+template <> __device__ inline const EoTable<4>& eo_table<4>() { return kEo4; }
+template <> __device__ inline const EoTable<5>& eo_table<5>() { return kEo5; }
+// Coefficient W (0 even, 1 odd, 2 middle) of row I, column J of operator A:
+// an immediate in FP32, a constant-memory load in FP64.
+template <typename T, class A, int I, int J, int W>
+__device__ inline T eo_coef() {
+  if constexpr (std::is_same<T, double>::value) {
+    using E = EoTable<A::kP>;
+    constexpr int c = E::C(A::kK);
+    constexpr int idx = E::base(A::kK) + I * E::stride(A::kK) +
+                        (W == 0 ? J : W == 1 ? c / 2 + J : 2 * (c / 2));
+    return eo_table<A::kP>().v[idx];
+  } else if constexpr (W == 0) {
+    return eo_even<T, A, I, J>();
+  } else if constexpr (W == 1) {
+    return eo_odd<T, A, I, J>();
+  } else {
+    return eo_mid<T, A, I>();
+  }
+}
 template <int C, typename T, int VC>
 __device__ inline void eo_split(const Pack<T, VC> (&x)[C], Pack<T, VC> (&xp)[(C + 1) / 2],
                                 Pack<T, VC> (&xm)[C / 2]) {
@@ -596,10 +669,10 @@ __device__ inline void eo_row(const Pack<T, VC> (&xp)[(A::C + 1) / 2],
                               Pack<T, VC>& o) {
   static_for<0, A::C / 2>([&](auto j) {
     constexpr int J = decltype(j)::value;
-    pack_fma(e, eo_even<T, A, I, J>(), xp[J]);
-    if constexpr (2 * I + 1 != A::R) pack_fma(o, eo_odd<T, A, I, J>(), xm[J]);
+    pack_fma(e, eo_coef<T, A, I, J, 0>(), xp[J]);
+    if constexpr (2 * I + 1 != A::R) pack_fma(o, eo_coef<T, A, I, J, 1>(), xm[J]);
   });
-  if constexpr (A::C % 2 == 1) pack_fma(e, eo_mid<T, A, I>(), xp[A::C / 2]);
+  if constexpr (A::C % 2 == 1) pack_fma(e, eo_coef<T, A, I, 0, 2>(), xp[A::C / 2]);
 }
 // The R outputs of one or more even-odd contractions: rows(i, e, o) adds the
 // sums of row i, put(i, y) receives output i (i an integral constant).
@@ -1767,6 +1840,20 @@ __global__ void k_copy_cols(int64_t N, int Bs, int off, int Bd, const double* __
   }
 }
 
+// dst[n][k] = src[n][idx[k]] (gather, Bd columns) or, with scatter,
+// dst[n][idx[k]] = src[n][k] (Bs columns of src into Bd-wide dst).
+__global__ void k_cols_by_index(int64_t N, int Bs, int Bd, int Bi, const int* __restrict__ idx,
+                                const double* __restrict__ src, double* __restrict__ dst,
+                                int scatter) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N * Bi;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t n = i / Bi;
+    const int k = (int)(i - n * Bi);
+    if (scatter) dst[n * Bd + idx[k]] = src[n * Bs + k];
+    else dst[n * Bd + k] = src[n * Bs + idx[k]];
+  }
+}
+
 __global__ void k_mul_q(int64_t nQ, int nq, const double* __restrict__ u,
                         const double* __restrict__ jxw, double* __restrict__ out) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nQ;
@@ -1904,6 +1991,25 @@ static void cell_gemm_p(const Geo& g, const CellOp<T>& op, const T* S, const TV*
     case 4: k_cell_gemm<T, TV, P, 4><<<grid, block, 0, s>>>(g, H, S, X, xn, xcol, Yc, B, cpw); break;
     case 5: k_cell_gemm<T, TV, P, 5><<<grid, block, 0, s>>>(g, H, S, X, xn, xcol, Yc, B, cpw); break;
   }
+}
+
+void upload_sf_constants() {
+  static std::mutex mu;
+  static bool done[64] = {};
+  int dev = 0;
+  IDFT_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) throw Error(IDFT_EHIP, "device index out of range");
+  std::lock_guard<std::mutex> lk(mu);
+  if (done[dev]) return;
+  static constexpr EoTable<2> t2 = make_eo_table<2>();
+  static constexpr EoTable<3> t3 = make_eo_table<3>();
+  static constexpr EoTable<4> t4 = make_eo_table<4>();
+  static constexpr EoTable<5> t5 = make_eo_table<5>();
+  IDFT_HIP(hipMemcpyToSymbol(HIP_SYMBOL(kEo2), &t2, sizeof(t2), 0, hipMemcpyHostToDevice));
+  IDFT_HIP(hipMemcpyToSymbol(HIP_SYMBOL(kEo3), &t3, sizeof(t3), 0, hipMemcpyHostToDevice));
+  IDFT_HIP(hipMemcpyToSymbol(HIP_SYMBOL(kEo4), &t4, sizeof(t4), 0, hipMemcpyHostToDevice));
+  IDFT_HIP(hipMemcpyToSymbol(HIP_SYMBOL(kEo5), &t5, sizeof(t5), 0, hipMemcpyHostToDevice));
+  done[dev] = true;
 }
 
 int64_t cell_scratch_elems(const Geo& g, int B) { return g.ncell * (int64_t)ne_of(g.p) * B; }
@@ -2261,6 +2367,16 @@ void launch_copy_done(int B, const double* st, int* flags, hipStream_t s) {
 void launch_copy_cols(int64_t N, int Bs, int off, int Bd, const double* src, double* dst,
                       hipStream_t s) {
   k_copy_cols<<<grid_for(N * Bd), kThreads, 0, s>>>(N, Bs, off, Bd, src, dst);
+  IDFT_HIP(hipGetLastError());
+}
+void launch_gather_cols(int64_t N, int Bs, const double* src, int Bd, const int* idx, double* dst,
+                        hipStream_t s) {
+  k_cols_by_index<<<grid_for(N * Bd), kThreads, 0, s>>>(N, Bs, Bd, Bd, idx, src, dst, 0);
+  IDFT_HIP(hipGetLastError());
+}
+void launch_scatter_cols(int64_t N, int Bs, const double* src, int Bd, const int* idx, double* dst,
+                         hipStream_t s) {
+  k_cols_by_index<<<grid_for(N * Bs), kThreads, 0, s>>>(N, Bs, Bd, Bs, idx, src, dst, 1);
   IDFT_HIP(hipGetLastError());
 }
 void launch_mul_q(int64_t nQ, int nq, const double* u, const double* jxw, double* out,

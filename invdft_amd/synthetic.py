@@ -47,6 +47,8 @@ CONFIGS = {
                            atoms="random8"),
     "gradcheck_12c_p4": dict(n_cells=12, degree=4, half_width=12.0, n_eig=6, n_electrons=8.0,
                              atoms=[(3.0, (0.0, 0.0, 0.0)), (1.0, (0.0, 0.0, 3.0))]),
+    "gradcheck_32c_p5": dict(n_cells=32, degree=5, half_width=12.0, n_eig=32, n_electrons=48.0,
+                             atoms="random8"),
 }
 
 

@@ -1693,3 +1693,672 @@ extern "C" int idft_bfgs_solve(idft_ctx* c, int32_t ls_type, int32_t max_iter, d
   if (n_done) *n_done = iter;
   return IDFT_OK;
 }
+
+// ============================================================ set-up stage
+// include/idft_setup.h on the CPU (test infrastructure, like the rest of this
+// file): the atomic-orbital target density as gaussianFunctionManager.cpp
+// evaluates it (spherical coordinates, associated Legendre functions), the
+// loss weights, the Poisson solve and the initial v_xc guess with assembled
+// cell matrices and plain CG.
+#include "../include/idft_setup.h"
+
+#include <fstream>
+#include <sstream>
+
+struct idft_ao_basis {
+  struct Fn {
+    double c[3];
+    int l = 0, m = 0;
+    vector<double> alpha, cn;  // exponents, coefficient * primitive norm
+    double norm = 1.0;         // basisNormConst
+  };
+  vector<Fn> f;
+};
+
+namespace {
+
+const double kPi = 3.14159265358979323846;
+// DFT-FE's Angstrom -> Bohr conversion (C_AngToBohr).
+const double kAngToBohr = 1.0 / 0.52917721067;
+
+double dfact(int n) {  // doubleFactorial (gaussianFunctionManager.cpp:133-137)
+  double r = 1.0;
+  for (int k = n; k > 1; k -= 2) r *= k;
+  return r;
+}
+// getNormConsts (:251-262)
+double prim_norm(double a, int l) {
+  const double ov = dfact(2 * l + 1) * std::sqrt(kPi) / (std::pow(2.0, 2 * l + 3.5) * std::pow(a, l + 1.5));
+  return 1.0 / std::sqrt(ov);
+}
+// Blm / Clm (:271-287) and Dm (:264-269)
+double blm(int l, int m) { return m == 0 ? std::sqrt((2.0 * l + 1) / 2.0) : blm(l, m - 1) / std::sqrt((l - m + 1.0) * (l + m)); }
+double dm(int m) { return m == 0 ? 1.0 / std::sqrt(2 * kPi) : 1.0 / std::sqrt(kPi); }
+// Associated Legendre P_l^m(x), m >= 0, without the Condon-Shortley phase
+// (Plm, :310-320), by the standard upward recurrence in l.
+double plm(int l, int m, double x) {
+  if (m > l) return 0.0;
+  const double s = std::sqrt(std::max(0.0, 1.0 - x * x));
+  double pmm = dfact(2 * m - 1) * std::pow(s, m);
+  if (l == m) return pmm;
+  double pm1 = x * (2 * m + 1) * pmm;
+  if (l == m + 1) return pm1;
+  double pl = 0.0;
+  for (int k = m + 2; k <= l; ++k) {
+    pl = (x * (2 * k - 1) * pm1 - (k + m - 1) * pmm) / (k - m);
+    pmm = pm1;
+    pm1 = pl;
+  }
+  return pl;
+}
+double fact(int n) { double r = 1.0; for (int k = 2; k <= n; ++k) r *= k; return r; }
+// dPlmDTheta (:322-350)
+double dplm_dtheta(int l, int m, double theta) {
+  const double ct = std::cos(theta);
+  if (std::abs(m) > l || l == 0) return 0.0;
+  if (m < 0) {
+    const int am = -m;
+    return std::pow(-1.0, m) * fact(l - am) / fact(l + am) * dplm_dtheta(l, am, theta);
+  }
+  if (m == 0) return -plm(l, 1, ct);
+  if (m == l) return l * plm(l, l - 1, ct);
+  return 0.5 * ((l + m) * (l - m + 1) * plm(l, m - 1, ct) - plm(l, m + 1, ct));
+}
+// Qm, dQm/dphi (:289-308)
+double qm(int m, double phi) { return m > 0 ? std::cos(m * phi) : m == 0 ? 1.0 : std::sin(-m * phi); }
+double dqm(int m, double phi) { return m > 0 ? -m * std::sin(m * phi) : m == 0 ? 0.0 : -m * std::cos(-m * phi); }
+// convertCartesianToSpherical (:147-172)
+void to_spherical(const double* d, double& r, double& th, double& ph) {
+  r = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+  th = 0.0; ph = 0.0;
+  if (r == 0) return;
+  th = std::acos(d[2] / r);
+  if (std::fabs(th) >= 1e-12 && std::fabs(th - kPi) >= 1e-12) ph = std::atan2(d[1], d[0]);
+}
+// gfRadialPartDerivative (:385-394)
+double radial_der(double r, double a, int l, int k) {
+  if (k == 0) return l >= 0 ? std::pow(r, l) * std::exp(-a * r * r) : 0.0;
+  return l * radial_der(r, a, l - 1, k - 1) - 2 * a * radial_der(r, a, l + 1, k - 1);
+}
+// evaluateBasisValue (:416-446)
+double ao_value(const idft_ao_basis::Fn& b, const double* x) {
+  const double d[3] = {x[0] - b.c[0], x[1] - b.c[1], x[2] - b.c[2]};
+  double r, th, ph;
+  to_spherical(d, r, th, ph);
+  double rad = 0.0;
+  for (size_t k = 0; k < b.alpha.size(); ++k) rad += b.cn[k] * std::pow(r, b.l) * std::exp(-b.alpha[k] * r * r);
+  const int am = std::abs(b.m);
+  return rad * blm(b.l, am) * dm(b.m) * plm(b.l, am, std::cos(th)) * qm(b.m, ph) * b.norm;
+}
+// getSphericalGaussianGradient (:449-576) summed over the contraction
+// (getBasisFunctionGradient, :1633-1663).
+void ao_gradient(const idft_ao_basis::Fn& b, const double* x, double* g) {
+  const double d[3] = {x[0] - b.c[0], x[1] - b.c[1], x[2] - b.c[2]};
+  double r, th, ph;
+  to_spherical(d, r, th, ph);
+  const int l = b.l, m = b.m, am = std::abs(m);
+  const double C = blm(l, am) * dm(m);
+  g[0] = g[1] = g[2] = 0.0;
+  for (size_t k = 0; k < b.alpha.size(); ++k) {
+    const double a = b.alpha[k];
+    double v[3] = {0, 0, 0};
+    if (r < 1e-15) {
+      if (l == 1) v[m == 1 ? 0 : m == -1 ? 1 : 2] = C;
+    } else if (std::fabs(th) < 1e-12 || std::fabs(th - kPi) < 1e-12) {
+      const double R = radial_der(r, a, l, 0), dR = radial_der(r, a, l, 1);
+      const double ct = std::cos(th), P = plm(l, am, ct);
+      const double sgn = std::fabs(th) < 1e-12 ? 1.0 : std::pow(-1.0, l + 1);
+      if (m == 0) v[2] = C * dR * P * ct;
+      else if (m == 1) v[0] = C * (R / r) * l * (l + 1) / 2.0 * sgn;
+      else if (m == -1) v[1] = C * (R / r) * l * (l + 1) / 2.0 * sgn;
+    } else {
+      const double R = radial_der(r, a, l, 0), dR = radial_der(r, a, l, 1);
+      const double ct = std::cos(th), st = std::sin(th);
+      const double P = plm(l, am, ct), dP = dplm_dtheta(l, am, th), Q = qm(m, ph), dQ = dqm(m, ph);
+      const double J[3][3] = {{st * std::cos(ph), ct * std::cos(ph) / r, -std::sin(ph) / (r * st)},
+                              {st * std::sin(ph), ct * std::sin(ph) / r, std::cos(ph) / (r * st)},
+                              {ct, -st / r, 0.0}};
+      const double pd[3] = {dR * P * Q, R * dP * Q, R * P * dQ};
+      for (int i = 0; i < 3; ++i) v[i] = C * (J[i][0] * pd[0] + J[i][1] * pd[1] + J[i][2] * pd[2]);
+    }
+    for (int i = 0; i < 3; ++i) g[i] += b.cn[k] * v[i];
+  }
+  for (int i = 0; i < 3; ++i) g[i] *= b.norm;
+}
+
+bool is_number(const std::string& w, double& v) {
+  char* end = nullptr;
+  v = std::strtod(w.c_str(), &end);
+  return end && *end == '\0' && !w.empty();
+}
+
+std::string join_dir(const char* dir, const std::string& f) {
+  if (!dir || !*dir || (!f.empty() && f[0] == '/')) return f;
+  return std::string(dir) + "/" + f;
+}
+
+// readGaussianFiles (:720-825): shells of one basis file.
+struct Shell { int l; vector<double> a, c; };
+bool read_shells(const std::string& path, vector<Shell>& out) {
+  std::ifstream in(path);
+  if (!in) return false;
+  std::string line;
+  std::getline(in, line);  // first line ignored
+  const std::string lch = "SPDFGHI";
+  while (std::getline(in, line)) {
+    std::istringstream ls(line);
+    std::string w;
+    if (!(ls >> w)) continue;
+    for (auto& ch : w) ch = (char)std::toupper(ch);
+    if (w.find_first_not_of(lch) != std::string::npos) return false;
+    std::string ns;
+    if (!(ls >> ns)) return false;
+    char* end = nullptr;
+    const long n = std::strtol(ns.c_str(), &end, 10);
+    if (!end || *end != '\0' || n < 1) return false;
+    vector<double> a(n);
+    vector<vector<double>> c(n, vector<double>(w.size()));
+    for (long i = 0; i < n; ++i) {
+      if (!std::getline(in, line) || line.empty()) return false;
+      std::istringstream lc(line);
+      std::string t;
+      if (!(lc >> t) || !is_number(t, a[i])) return false;
+      for (size_t j = 0; j < w.size(); ++j)
+        if (!(lc >> t) || !is_number(t, c[i][j])) return false;
+    }
+    for (size_t j = 0; j < w.size(); ++j) {
+      Shell s;
+      s.l = (int)lch.find(w[j]);
+      s.a = a;
+      for (long i = 0; i < n; ++i) s.c.push_back(c[i][j]);
+      out.push_back(s);
+    }
+  }
+  return true;
+}
+
+// Parent quadrature point coordinates of cell e, point q.
+void quad_xyz(const Mesh& m, long e, int q, double* x) {
+  const long cx = e % m.nc, cy = (e / m.nc) % m.nc, cz = e / ((long)m.nc * m.nc);
+  const int qx = q % m.q1, qy = (q / m.q1) % m.q1, qz = q / (m.q1 * m.q1);
+  x[0] = -m.L + cx * m.h + 0.5 * m.h * (m.qx[qx] + 1.0);
+  x[1] = -m.L + cy * m.h + 0.5 * m.h * (m.qx[qy] + 1.0);
+  x[2] = -m.L + cz * m.h + 0.5 * m.h * (m.qx[qz] + 1.0);
+}
+void node_xyz(const Mesh& m, long n, double* x) {
+  const long i = n % m.N1, j = (n / m.N1) % m.N1, k = n / ((long)m.N1 * m.N1);
+  const int ai = (int)(i % m.p), aj = (int)(j % m.p), ak = (int)(k % m.p);
+  const long ci = i / m.p, cj = j / m.p, ck = k / m.p;
+  x[0] = -m.L + ci * m.h + 0.5 * m.h * (m.gx[ai] + 1.0);
+  x[1] = -m.L + cj * m.h + 0.5 * m.h * (m.gx[aj] + 1.0);
+  x[2] = -m.L + ck * m.h + 0.5 * m.h * (m.gx[ak] + 1.0);
+}
+
+// y = sum_e P_e^T A_e P_e x over all nodes (A_e: [ne][ne]) with rows and
+// columns of boundary nodes dropped (zero) when `interior`.
+void cell_apply(const Mesh& m, const vector<double>& Ae, const vector<double>& x, vector<double>& y,
+                bool interior) {
+  const int ne = m.ne;
+  y.assign(m.N, 0.0);
+  for (int col = 0; col < 8; ++col) {
+#pragma omp parallel for schedule(static)
+    for (long e = 0; e < m.ncell; ++e) {
+      if (m.colour(e) != col) continue;
+      const long* nodes = &m.cellNodes[(size_t)e * ne];
+      for (int a = 0; a < ne; ++a) {
+        if (interior && m.boundary(nodes[a])) continue;
+        double s = 0.0;
+        for (int b = 0; b < ne; ++b)
+          if (!(interior && m.boundary(nodes[b]))) s += Ae[(size_t)a * ne + b] * x[nodes[b]];
+        y[nodes[a]] += s;
+      }
+    }
+  }
+}
+// f_a = sum_e sum_q N_a(q) w[e][q]  (integration against the parent basis)
+void integrate_basis(const Mesh& m, const vector<double>& w, vector<double>& f) {
+  const int ne = m.ne, nq = m.nq;
+  f.assign(m.N, 0.0);
+  for (int col = 0; col < 8; ++col) {
+#pragma omp parallel for schedule(static)
+    for (long e = 0; e < m.ncell; ++e) {
+      if (m.colour(e) != col) continue;
+      const long* nodes = &m.cellNodes[(size_t)e * ne];
+      for (int a = 0; a < ne; ++a) {
+        double s = 0.0;
+        for (int q = 0; q < nq; ++q) s += m.Nq[(size_t)q * ne + a] * w[(size_t)e * nq + q];
+        f[nodes[a]] += s;
+      }
+    }
+  }
+}
+// Preconditioned CG on the interior nodes: A x = b, D the diagonal
+// preconditioner, x0 = 0; stops at ||r||_D <= tol ||b||_D.
+template <typename Apply>
+int pcg(const Mesh& m, Apply&& A, const vector<double>& b, const vector<double>& D, double tol,
+        int maxit, vector<double>& x, double* relres) {
+  const long N = m.N;
+  x.assign(N, 0.0);
+  vector<double> r = b, z(N), p(N), q;
+  auto dot = [&](const vector<double>& u, const vector<double>& v) {
+    double s = 0.0;
+#pragma omp parallel for reduction(+ : s)
+    for (long i = 0; i < N; ++i) s += u[i] * v[i];
+    return s;
+  };
+  for (long i = 0; i < N; ++i) z[i] = D[i] * r[i];
+  p = z;
+  double rz = dot(r, z);
+  const double bn = std::sqrt(std::max(rz, 0.0));
+  int it = 0;
+  double rn = bn;
+  while (it < maxit && rn > tol * bn && bn > 0.0) {
+    A(p, q);
+    const double alpha = rz / dot(p, q);
+    for (long i = 0; i < N; ++i) { x[i] += alpha * p[i]; r[i] -= alpha * q[i]; }
+    for (long i = 0; i < N; ++i) z[i] = D[i] * r[i];
+    const double rz2 = dot(r, z);
+    rn = std::sqrt(std::max(rz2, 0.0));
+    const double beta = rz2 / rz;
+    rz = rz2;
+    for (long i = 0; i < N; ++i) p[i] = z[i] + beta * p[i];
+    ++it;
+  }
+  if (relres) *relres = bn > 0.0 ? rn / bn : 0.0;
+  return it;
+}
+// Cell mass matrix M_e = sum_q N_a N_b JxW (consistent mass at the parent quadrature).
+vector<double> cell_mass(const Mesh& m) {
+  const int ne = m.ne, nq = m.nq;
+  vector<double> Me((size_t)ne * ne, 0.0);
+  for (int q = 0; q < nq; ++q)
+    for (int a = 0; a < ne; ++a)
+      for (int b = 0; b < ne; ++b) Me[(size_t)a * ne + b] += m.Nq[(size_t)q * ne + a] * m.Nq[(size_t)q * ne + b] * m.JxW[q];
+  return Me;
+}
+// out[e][q] = sum_a N_a(q) x[node_a]
+void nodal_to_quad(const Mesh& m, const vector<double>& x, double* out) {
+  const int ne = m.ne, nq = m.nq;
+#pragma omp parallel for schedule(static)
+  for (long e = 0; e < m.ncell; ++e) {
+    const long* nodes = &m.cellNodes[(size_t)e * ne];
+    for (int q = 0; q < nq; ++q) {
+      double s = 0.0;
+      for (int a = 0; a < ne; ++a) s += m.Nq[(size_t)q * ne + a] * x[nodes[a]];
+      out[(size_t)e * nq + q] = s;
+    }
+  }
+}
+vector<double> diag_of(const Mesh& m, const vector<double>& Ae) {
+  vector<double> d(m.N, 0.0);
+  for (long e = 0; e < m.ncell; ++e)
+    for (int a = 0; a < m.ne; ++a) d[m.cellNodes[(size_t)e * m.ne + a]] += Ae[(size_t)a * m.ne + a];
+  return d;
+}
+
+// ---- exchange-correlation potentials (libxc is not in the reference; the
+// published forms are restated).
+// Slater exchange of the total density n: v_x = -(3 n / pi)^{1/3}.
+double slater_vx(double n) { return n > 1e-30 ? -std::cbrt(3.0 * n / kPi) : 0.0; }
+// van Leeuwen-Baerends 1994 exchange potential (beta = 0.05) of the total
+// density n with sigma = |grad n|^2, per spin channel n/2.
+double lb94_vx(double n, double sigma) {
+  if (n <= 1e-30) return 0.0;
+  const double beta = 0.05, ns = 0.5 * n;
+  const double x = 0.5 * std::sqrt(std::max(sigma, 0.0)) / std::pow(ns, 4.0 / 3.0);
+  return slater_vx(n) - beta * std::cbrt(ns) * x * x / (1.0 + 3.0 * beta * x * std::asinh(x));
+}
+// Perdew-Wang 1992 correlation (spin-unpolarised, Table I parameters of the
+// paper): eps_c(r_s) = -2A(1 + a1 r_s) ln(1 + 1/(2A(b1 rs^1/2 + b2 rs + b3 rs^3/2 + b4 rs^2))),
+// v_c = eps_c - (r_s/3) d eps_c / d r_s.
+double pw92_vc(double n) {
+  if (n <= 1e-30) return 0.0;
+  const double A = 0.031091, a1 = 0.21370, b1 = 7.5957, b2 = 3.5876, b3 = 1.6382, b4 = 0.49294;
+  const double rs = std::cbrt(3.0 / (4.0 * kPi * n)), srs = std::sqrt(rs);
+  const double Q0 = -2.0 * A * (1.0 + a1 * rs);
+  const double Q1 = 2.0 * A * (b1 * srs + b2 * rs + b3 * rs * srs + b4 * rs * rs);
+  const double Q1p = A * (b1 / srs + 2.0 * b2 + 3.0 * b3 * srs + 4.0 * b4 * rs);
+  const double lg = std::log(1.0 + 1.0 / Q1);
+  const double eps = Q0 * lg;
+  const double deps = -2.0 * A * a1 * lg - Q0 * Q1p / (Q1 * Q1 + Q1);
+  return eps - rs / 3.0 * deps;
+}
+
+}  // namespace
+
+extern "C" {
+
+int idft_ao_basis_read(const char* coords_file, const char* basis_dir, int32_t angstrom,
+                       idft_ao_basis** out) {
+  if (!coords_file || !out) return IDFT_EINVAL;
+  *out = nullptr;
+  std::ifstream in(coords_file);
+  if (!in) return IDFT_EINVAL;
+  auto* b = new idft_ao_basis();
+  std::map<std::string, vector<Shell>> cache;
+  std::string line;
+  while (std::getline(in, line)) {
+    std::istringstream ls(line);
+    std::string sym, xs, ys, zs, file;
+    if (!(ls >> sym)) continue;
+    double c[3];
+    if (!(ls >> xs >> ys >> zs >> file) || !is_number(xs, c[0]) || !is_number(ys, c[1]) ||
+        !is_number(zs, c[2])) { delete b; return IDFT_EINVAL; }
+    if (angstrom) for (double& v : c) v *= kAngToBohr;
+    const std::string path = join_dir(basis_dir, file);
+    if (!cache.count(path)) {
+      vector<Shell> sh;
+      if (!read_shells(path, sh)) { delete b; return IDFT_EINVAL; }
+      cache[path] = sh;
+    }
+    for (const Shell& s : cache[path]) {
+      vector<int> ms;
+      if (s.l == 1) ms = {1, -1, 0};
+      else for (int mm = -s.l; mm <= s.l; ++mm) ms.push_back(mm);
+      for (int mm : ms) {
+        idft_ao_basis::Fn f;
+        std::copy(c, c + 3, f.c);
+        f.l = s.l; f.m = mm; f.alpha = s.a;
+        for (size_t k = 0; k < s.a.size(); ++k) f.cn.push_back(s.c[k] * prim_norm(s.a[k], s.l));
+        b->f.push_back(f);
+      }
+    }
+  }
+  if (b->f.empty()) { delete b; return IDFT_EINVAL; }
+  *out = b;
+  return IDFT_OK;
+}
+
+void idft_ao_basis_destroy(idft_ao_basis* b) { delete b; }
+
+int idft_ao_basis_size(const idft_ao_basis* b, int32_t* n) {
+  if (!b || !n) return IDFT_EINVAL;
+  *n = (int32_t)b->f.size();
+  return IDFT_OK;
+}
+
+int idft_ao_read_matrix(const char* path, int32_t n, double* out) {
+  if (!path || !out || n < 1) return IDFT_EINVAL;
+  std::ifstream in(path);
+  if (!in) return IDFT_EINVAL;
+  std::string line, w;
+  int row = 0;
+  while (std::getline(in, line)) {
+    if (line.empty()) return IDFT_EINVAL;
+    std::istringstream ls(line);
+    int col = 0;
+    while (ls >> w) {
+      double v;
+      if (!is_number(w, v) || row >= n || col >= n) return IDFT_EINVAL;
+      out[(size_t)row * n + col++] = v;
+    }
+    if (col != n) return IDFT_EINVAL;
+    ++row;
+  }
+  return row == n ? IDFT_OK : IDFT_EINVAL;
+}
+
+int idft_ao_values(const idft_ao_basis* b, int64_t np, const double* xyz, double* vals) {
+  if (!b || !xyz || !vals || np < 0) return IDFT_EINVAL;
+  const size_t nb = b->f.size();
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < (long)nb; ++i)
+    for (int64_t p = 0; p < np; ++p) vals[(size_t)i * np + p] = ao_value(b->f[i], xyz + 3 * p);
+  return IDFT_OK;
+}
+
+int idft_ao_density_quad(const idft_desc* d, idft_ao_basis* b, int32_t normalize, int32_t n_spin,
+                         const double* P, double* rho_q, double* grad_q, double* s_diag) {
+  if (!desc_ok(d) || !b || !P || !rho_q || n_spin < 1 || n_spin > 2) return IDFT_EINVAL;
+  Mesh m; m.init(*d);
+  const long nQ = m.ncell * m.nq;
+  const int nb = (int)b->f.size();
+  for (auto& f : b->f) f.norm = 1.0;
+  // self-overlaps on this quadrature (evaluateForQuad, :1252-1305)
+  vector<double> S(nb, 0.0);
+#pragma omp parallel
+  {
+    vector<double> loc(nb, 0.0);
+#pragma omp for schedule(static)
+    for (long i = 0; i < nQ; ++i) {
+      double x[3];
+      quad_xyz(m, i / m.nq, (int)(i % m.nq), x);
+      const double w = m.JxW[i % m.nq];
+      for (int k = 0; k < nb; ++k) { const double v = ao_value(b->f[k], x); loc[k] += v * v * w; }
+    }
+#pragma omp critical
+    for (int k = 0; k < nb; ++k) S[k] += loc[k];
+  }
+  if (s_diag) std::copy(S.begin(), S.end(), s_diag);
+  if (normalize)
+    for (int k = 0; k < nb; ++k) b->f[k].norm = 1.0 / std::sqrt(S[k]);
+  // getRhoValue (:1387-1430) and getRhoGradient (:1481-1516)
+#pragma omp parallel
+  {
+    vector<double> phi(nb), gphi(3 * (size_t)nb);
+#pragma omp for schedule(static)
+    for (long i = 0; i < nQ; ++i) {
+      double x[3];
+      quad_xyz(m, i / m.nq, (int)(i % m.nq), x);
+      for (int k = 0; k < nb; ++k) {
+        phi[k] = ao_value(b->f[k], x);
+        if (grad_q) ao_gradient(b->f[k], x, &gphi[3 * k]);
+      }
+      for (int s = 0; s < n_spin; ++s) {
+        const double* Ps = P + (size_t)s * nb * nb;
+        double val = 0.0, g[3] = {0, 0, 0};
+        for (int a = 0; a < nb; ++a) {
+          val += Ps[(size_t)a * nb + a] * phi[a] * phi[a];
+          for (int c = a + 1; c < nb; ++c) val += 2.0 * Ps[(size_t)a * nb + c] * phi[a] * phi[c];
+          if (grad_q)
+            for (int c = 0; c < nb; ++c) {
+              const double pac = Ps[(size_t)std::min(a, c) * nb + std::max(a, c)];
+              for (int k = 0; k < 3; ++k) g[k] += pac * (phi[a] * gphi[3 * c + k] + phi[c] * gphi[3 * a + k]);
+            }
+        }
+        rho_q[(size_t)s * nQ + i] = val;
+        if (grad_q)
+          for (int k = 0; k < 3; ++k) grad_q[((size_t)s * nQ + i) * 3 + k] = g[k];
+      }
+    }
+  }
+  return IDFT_OK;
+}
+
+int idft_loss_weights(int64_t n, const double* rho, double sf, double a1, double a2, double tau,
+                      double* w) {
+  if (n < 0 || !rho || !w) return IDFT_EINVAL;
+  for (int64_t i = 0; i < n; ++i) {
+    const double r = sf * rho[i];
+    w[i] = 1.0 / (std::pow(r, a1) + tau) + std::pow(r, a2);
+  }
+  return IDFT_OK;
+}
+
+int idft_poisson_solve(const idft_desc* d, const double* cq, const double* bc, double tol,
+                       int32_t maxit, double* phi_q, double* phi_nodal, idft_poisson_stats* st) {
+  if (!desc_ok(d) || !cq || !(tol > 0) || maxit < 1) return IDFT_EINVAL;
+  Mesh m; m.init(*d);
+  const long nQ = m.ncell * m.nq;
+  // charge moments about the origin
+  double Q = 0.0, dip[3] = {0, 0, 0};
+  for (long i = 0; i < nQ; ++i) {
+    double x[3];
+    quad_xyz(m, i / m.nq, (int)(i % m.nq), x);
+    const double w = cq[i] * m.JxW[i % m.nq];
+    Q += w;
+    for (int k = 0; k < 3; ++k) dip[k] += w * x[k];
+  }
+  vector<double> g(m.N, 0.0);
+  for (long n = 0; n < m.N; ++n) {
+    if (!m.boundary(n)) continue;
+    if (bc) { g[n] = bc[n]; continue; }
+    double x[3];
+    node_xyz(m, n, x);
+    const double r = std::sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+    g[n] = Q / r + (dip[0] * x[0] + dip[1] * x[1] + dip[2] * x[2]) / (r * r * r);
+  }
+  // K phi_I = 4 pi int N c - K g on the interior nodes
+  vector<double> w4((size_t)nQ), f, Kg;
+  for (long i = 0; i < nQ; ++i) w4[i] = 4.0 * kPi * cq[i] * m.JxW[i % m.nq];
+  integrate_basis(m, w4, f);
+  cell_apply(m, m.Ke, g, Kg, false);
+  vector<double> b(m.N, 0.0), Dinv(m.N, 0.0);
+  const vector<double> Kd = diag_of(m, m.Ke);
+  for (long n = 0; n < m.N; ++n)
+    if (!m.boundary(n)) { b[n] = f[n] - Kg[n]; Dinv[n] = 1.0 / Kd[n]; }
+  vector<double> x;
+  double rr = 0.0;
+  const int it = pcg(m, [&](const vector<double>& u, vector<double>& v) { cell_apply(m, m.Ke, u, v, true); },
+                     b, Dinv, tol, maxit, x, &rr);
+  for (long n = 0; n < m.N; ++n) x[n] = m.boundary(n) ? g[n] : x[n];
+  if (phi_nodal) std::copy(x.begin(), x.end(), phi_nodal);
+  if (phi_q) nodal_to_quad(m, x, phi_q);
+  if (st) {
+    st->iterations = it; st->rel_residual = rr; st->charge = Q;
+    for (int k = 0; k < 3; ++k) st->dipole[k] = dip[k];
+  }
+  return it >= maxit && rr > tol ? IDFT_ENOCONV : IDFT_OK;
+}
+
+}  // extern "C"
+
+// ---- initial v_xc guess (setInitialPotL2Proj, InverseDFTEngine.cpp:1044-1431)
+extern "C" {
+
+void idft_guess_params_default(idft_guess_params* gp) {
+  if (!gp) return;
+  gp->lb94 = 1; gp->tau_vx = 1e-2; gp->fermi_amaldi = 1; gp->tau_fa = 1e-2;
+  gp->factor_fa = 1.0; gp->factor_lda = 0.0; gp->spin_factor = 2.0; gp->n_electrons = 0.0;
+}
+
+int idft_xc_potentials(int64_t n, const double* r, const double* sigma, int32_t lb94, double* vx,
+                       double* vc) {
+  if (n < 0 || !r || !vx || !vc || (lb94 && !sigma)) return IDFT_EINVAL;
+  for (int64_t i = 0; i < n; ++i) {
+    vx[i] = lb94 ? lb94_vx(r[i], sigma[i]) : slater_vx(r[i]);
+    vc[i] = pw92_vc(r[i]);
+  }
+  return IDFT_OK;
+}
+
+int idft_l2_project_parent(const idft_desc* d, const double* v_q, double* x_nodal) {
+  if (!desc_ok(d) || !v_q || !x_nodal) return IDFT_EINVAL;
+  Mesh m; m.init(*d);
+  const long nQ = m.ncell * m.nq;
+  const vector<double> Me = cell_mass(m);
+  vector<double> w((size_t)nQ), f;
+  for (long i = 0; i < nQ; ++i) w[i] = v_q[i] * m.JxW[i % m.nq];
+  integrate_basis(m, w, f);
+  const vector<double> Md = diag_of(m, Me);
+  vector<double> Dinv(m.N, 0.0);
+  for (long n = 0; n < m.N; ++n)
+    if (!m.boundary(n)) Dinv[n] = 1.0 / Md[n]; else f[n] = 0.0;
+  vector<double> xp;
+  double rr = 0.0;
+  pcg(m, [&](const vector<double>& u, vector<double>& y) { cell_apply(m, Me, u, y, true); }, f, Dinv,
+      1e-14, 2000, xp, &rr);
+  std::copy(xp.begin(), xp.end(), x_nodal);
+  return IDFT_OK;
+}
+
+// Values of the parent field at the 4 child Gauss points per parent axis,
+// times scale, integrated against the child Q1 basis, then the child
+// consistent mass (assembled element by element, Gauss 2) solved by CG.
+int idft_parent_to_child(const idft_desc* d, const double* x_nodal, double scale, double* out) {
+  if (!desc_ok(d) || !x_nodal || !out) return IDFT_EINVAL;
+  Mesh m; m.init(*d);
+  const long NV1 = m.NV1, NV = m.NV;
+  const double jc = std::pow(0.25 * m.h, 3);
+  vector<double> rhs(NV, 0.0);
+  for (long e = 0; e < m.ncell; ++e) {
+    const long cx = e % m.nc, cy = (e / m.nc) % m.nc, cz = e / ((long)m.nc * m.nc);
+    const long* nodes = &m.cellNodes[(size_t)e * m.ne];
+    for (int z = 0; z < 4; ++z) for (int y = 0; y < 4; ++y) for (int x = 0; x < 4; ++x) {
+      double val = 0.0;
+      for (int c = 0; c < m.n1; ++c) for (int b = 0; b < m.n1; ++b) for (int a = 0; a < m.n1; ++a)
+        val += m.Lc[x * m.n1 + a] * m.Lc[y * m.n1 + b] * m.Lc[z * m.n1 + c] * x_nodal[nodes[(c * m.n1 + b) * m.n1 + a]];
+      val *= scale;
+      for (int kc = 0; kc < 2; ++kc) for (int kb = 0; kb < 2; ++kb) for (int ka = 0; ka < 2; ++ka) {
+        const long i = 2 * cx + m.ccell[x] + ka, j = 2 * cy + m.ccell[y] + kb, k = 2 * cz + m.ccell[z] + kc;
+        rhs[(k * NV1 + j) * NV1 + i] += m.cw[x][ka] * m.cw[y][kb] * m.cw[z][kc] * val * jc;
+      }
+    }
+  }
+  // child element mass from the child basis at its own Gauss points
+  const double g2 = 1.0 / std::sqrt(3.0), hc = 0.5 * m.h;
+  double me1[2][2] = {{0, 0}, {0, 0}};
+  for (int k = 0; k < 2; ++k) {
+    const double t = k ? g2 : -g2, n0 = 0.5 * (1 - t), n1v = 0.5 * (1 + t);
+    me1[0][0] += 0.5 * hc * n0 * n0; me1[0][1] += 0.5 * hc * n0 * n1v;
+    me1[1][0] += 0.5 * hc * n1v * n0; me1[1][1] += 0.5 * hc * n1v * n1v;
+  }
+  auto childMass = [&](const vector<double>& u, vector<double>& y) {
+    y.assign(NV, 0.0);
+    for (long ck = 0; ck < NV1 - 1; ++ck) for (long cj = 0; cj < NV1 - 1; ++cj) for (long ci = 0; ci < NV1 - 1; ++ci)
+      for (int kc = 0; kc < 2; ++kc) for (int kb = 0; kb < 2; ++kb) for (int ka = 0; ka < 2; ++ka) {
+        double s = 0.0;
+        for (int lc = 0; lc < 2; ++lc) for (int lb = 0; lb < 2; ++lb) for (int la = 0; la < 2; ++la)
+          s += me1[ka][la] * me1[kb][lb] * me1[kc][lc] * u[((ck + lc) * NV1 + cj + lb) * NV1 + ci + la];
+        y[((ck + kc) * NV1 + cj + kb) * NV1 + ci + ka] += s;
+      }
+  };
+  vector<double> xc(NV, 0.0), r = rhs, p, q, z(NV), dg(NV);
+  {
+    vector<double> e1(NV, 0.0);
+    // diagonal: product over axes of the 1D diagonal (end nodes have one element)
+    for (long n = 0; n < NV; ++n) {
+      const long i = n % NV1, j = (n / NV1) % NV1, k = n / (NV1 * NV1);
+      auto f1 = [&](long t) { return (t == 0 || t == NV1 - 1) ? me1[0][0] : 2.0 * me1[0][0]; };
+      dg[n] = f1(i) * f1(j) * f1(k);
+    }
+  }
+  for (long n = 0; n < NV; ++n) z[n] = r[n] / dg[n];
+  p = z;
+  double rz = 0.0;
+  for (long n = 0; n < NV; ++n) rz += r[n] * z[n];
+  const double bn = std::sqrt(std::max(rz, 0.0));
+  for (int it = 0; it < 2000 && std::sqrt(std::max(rz, 0.0)) > 1e-14 * bn && bn > 0; ++it) {
+    childMass(p, q);
+    double pq = 0.0;
+    for (long n = 0; n < NV; ++n) pq += p[n] * q[n];
+    const double al = rz / pq;
+    for (long n = 0; n < NV; ++n) { xc[n] += al * p[n]; r[n] -= al * q[n]; z[n] = r[n] / dg[n]; }
+    double rz2 = 0.0;
+    for (long n = 0; n < NV; ++n) rz2 += r[n] * z[n];
+    const double be = rz2 / rz;
+    rz = rz2;
+    for (long n = 0; n < NV; ++n) p[n] = z[n] + be * p[n];
+  }
+  std::copy(xc.begin(), xc.end(), out);
+  return IDFT_OK;
+}
+
+int idft_initial_vxc(const idft_desc* d, const idft_guess_params* gp, const double* rho_spin_q,
+                     const double* sigma_q, const double* vH_q, double* v_lda_q, double* vxc_child) {
+  if (!desc_ok(d) || !gp || !rho_spin_q || (gp->lb94 && !sigma_q) ||
+      (gp->fermi_amaldi && (!vH_q || !(gp->n_electrons > 0))))
+    return IDFT_EINVAL;
+  Mesh m; m.init(*d);
+  const long nQ = m.ncell * m.nq;
+  vector<double> v((size_t)nQ);
+  for (long i = 0; i < nQ; ++i) {
+    const double r = gp->spin_factor * rho_spin_q[i];
+    const double vx = gp->lb94 ? lb94_vx(r, sigma_q[i]) : slater_vx(r);
+    const double pre = r / (r + gp->tau_vx);
+    double val = (1.0 - pre) * vx + pre * (vx + pw92_vc(r));
+    if (gp->fermi_amaldi) {
+      const double pb = r / (r + gp->tau_fa);
+      val = pb * val + (1.0 - pb) * (-1.0 / gp->n_electrons) * gp->factor_fa * vH_q[i];
+    }
+    v[i] = val;
+  }
+  if (v_lda_q) std::copy(v.begin(), v.end(), v_lda_q);
+  if (!vxc_child) return IDFT_OK;
+  vector<double> xp(m.N);
+  int rc = idft_l2_project_parent(d, v.data(), xp.data());
+  if (rc) return rc;
+  return idft_parent_to_child(d, xp.data(), 1.0 - gp->factor_lda, vxc_child);
+}
+
+}  // extern "C"

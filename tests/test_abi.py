@@ -26,12 +26,13 @@ def hip_host():
 
 
 def test_hip_library_exports_every_declared_symbol(hip_host):
-    for name in _declared("idft.h") + _declared("idft_hip.h") + _declared("idft_io.h"):
+    for name in (_declared("idft.h") + _declared("idft_hip.h") + _declared("idft_io.h") +
+                 _declared("idft_setup.h")):
         assert hasattr(hip_host, name), name
 
 
-def test_oracle_exports_the_common_header(olib):
-    for name in _declared("idft.h"):
+def test_oracle_exports_the_common_headers(olib):
+    for name in _declared("idft.h") + _declared("idft_setup.h"):
         assert hasattr(olib, name), name
 
 
