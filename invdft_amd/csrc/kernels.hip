@@ -511,17 +511,22 @@ __global__ __launch_bounds__(kSymmThreads) void k_cell_symm(Geo g, const T* __re
 //
 // One work-group handles one cell and CG columns; an item is one line of the
 // cell's tensor along one axis for VC consecutive columns (VC-wide LDS
-// accesses).  Intermediates live in LDS as [entry][CG / VC] packs; the
-// x-line arrays have padded line strides (Q+1, N1+1 entries) so that the 64
-// lanes of a wave spread over the banks:
-//   1  x lines (c,b):   t1[c][b][qx] = Lq x,  ka = M x,  kb = D x   (x gathered from HBM)
-//   2  y lines:         t2[c][qy][qx] = Lq t1;  k1 = M ka,  k2 = D ka + M kb
-//   3  z lines:         u = w .* (Lq t2),  s2[c][qy][qx] = Lq^T u;  yk = D k1 + M k2
-//   4  y lines back:    s1[c][b][qx] = Lq^T s2
-//   5  x lines back:    Yc[e][(c,b,a)][j] = (Lq^T s1)[a] + (h/4) yk[a]
-// The regions r1 / r2 are reused: (t1, ka, kb) -> (s2, yk) in r1 and
-// (t2, k1, k2) -> s1 in r2, each rewritten only after the barrier that ends
-// the stage reading it.
+// accesses).  Intermediates live in LDS as [entry][CG / VC] packs, with padded
+// line strides (N1+1, Q+1 entries) so that the 64 lanes of a wave spread over
+// the banks.  The kinetic chain runs first and the x-line input stays in
+// registers for the potential chain, and the y and z stages overwrite their
+// input in place (each item reads its whole line before writing the same
+// positions), so only three regions are live:
+//   1  x lines (c,b):  ka = M x -> A,  kb = D x -> Bk      (x gathered from HBM)
+//   2  y lines (c,a):  k1 = M ka -> A,  k2 = D ka + M kb -> Bk
+//   3  z lines (b,a):  yk = D k1 + M k2 -> A
+//   4  x lines (c,b):  t1 = Lq x -> Bk                     (x from registers)
+//   5  y lines (c,qx): t2[c][qy][qx] = Lq t1 -> D
+//   6  z lines (qy,qx): s2 = Lq^T (w .* (Lq t2)) -> D
+//   7  y lines (c,qx): s1[c][b][qx] = Lq^T s2 -> D
+//   8  x lines (c,b):  Yc[e][(c,b,a)][j] = (Lq^T s1)[a] + (h/4) yk[a]
+// At p = 5 that is 1008 entries per column (52.6 KB -> 32 KB per work-group
+// of 4 FP64 columns: 5 work-groups per CU instead of 3).
 __host__ __device__ constexpr int cmax(int a, int b) { return a > b ? a : b; }
 
 template <typename T, int VC>
@@ -707,17 +712,22 @@ __global__ __launch_bounds__(kThreads) void k_cell_sf(Geo g, T hq, const T* __re
                                                       const double* __restrict__ xn,
                                                       const double* __restrict__ xcol,
                                                       T* __restrict__ Yc, int B) {
-  constexpr int N1 = P + 1, Q = P + 3, N2 = N1 * N1, NE = N2 * N1, Q2 = Q * Q, Q3 = Q2 * Q;
+  constexpr int N1 = P + 1, Q = P + 3, N2 = N1 * N1, NE = N2 * N1, Q2 = Q * Q;
   constexpr int QP = Q + 1, N1P = N1 + 1, NV = CG / VC;
-  constexpr int SZ1 = cmax(N2 * QP + 2 * N2 * N1P, N1 * Q2 + N2 * N1P);
-  constexpr int SZ2 = cmax(N1 * Q2 + 2 * NE, N2 * QP);
+  // x-line items per thread (lines x column packs over the work-group)
+  constexpr int KI = (N2 * NV + kThreads - 1) / kThreads;
   typedef Pack<T, VC> pk;
   typedef SfTab<P, 0> TL;
   typedef SfTab<P, 1> TLt;
   typedef SfTab<P, 2> TM;
   typedef SfTab<P, 3> TD;
-  __shared__ pk r1[SZ1 * NV];
-  __shared__ pk r2[SZ2 * NV];
+  // three regions, each [entry][NV] packs:
+  //   A  [(c,b)][N1P]      ka -> k1 -> yk            (kinetic chain, in place)
+  //   Bk [(c,b)][QP]       kb -> k2, then t1         (t1 once k2 is consumed)
+  //   D  [c][Q][QP]        t2 -> s2 -> s1            (potential chain, in place)
+  __shared__ pk rA[N2 * N1P * NV];
+  __shared__ pk rB[N2 * QP * NV];
+  __shared__ pk rD[N1 * Q * QP * NV];
   // (cell, column group) pairs dealt to the 8 XCDs in contiguous runs, the
   // groups of one cell at consecutive slots of one XCD, so that they share
   // that XCD's L2 for the cell's rows of X and of the cell scratch (each group
@@ -734,18 +744,14 @@ __global__ __launch_bounds__(kThreads) void k_cell_sf(Geo g, T hq, const T* __re
   const int bw = min(CG, B - c0);
   const int tid = threadIdx.x;
   const uint32_t cx = e % nc, cy = (e / nc) % nc, cz = e / (nc * nc);
-  pk* t1 = r1;                   // [N2][QP]
-  pk* ka = r1 + N2 * QP * NV;    // [N2][N1P]
-  pk* kb = ka + N2 * N1P * NV;   // [N2][N1P]
-  pk* t2 = r2;                   // [N1][Q][Q]
-  pk* k1 = r2 + N1 * Q2 * NV;    // [N1][N1][N1]
-  pk* k2 = k1 + NE * NV;         // [N1][N1][N1]
-  pk* s2 = r1;                   // [N1][Q][Q]
-  pk* yk = r1 + N1 * Q2 * NV;    // [N2][N1P]
-  pk* s1 = r2;                   // [N2][QP]
-  const T* we = W + e * (uint32_t)Q3;
-  // stage 1: x lines ln = c*N1 + b
-  for (int it = tid; it < N2 * NV; it += kThreads) {
+  const T* we = W + e * (uint32_t)(Q2 * Q);
+  // the input lines stay in registers from stage 1 to stage 4 (even-odd split)
+  pk keep_p[KI][(N1 + 1) / 2], keep_m[KI][N1 / 2];
+  // stage 1: x lines ln = c*N1 + b of the gathered input -> ka = M x, kb = D x
+#pragma unroll
+  for (int ki = 0; ki < KI; ++ki) {
+    const int it = tid + ki * kThreads;
+    if (it >= N2 * NV) break;
     const int jv = it % NV, ln = it / NV;
     pk v[N1];
     const uint32_t b = ln % N1, c = ln / N1;
@@ -790,58 +796,91 @@ __global__ __launch_bounds__(kThreads) void k_cell_sf(Geo g, T hq, const T* __re
         for (int a = 0; a < N1; ++a) v[a].v[u] = T(0);
       }
     }
-    pk vp[(N1 + 1) / 2], vm[N1 / 2];
-    eo_split(v, vp, vm);
-    eo_out<Q, T, VC>([&](auto i, pk& s, pk& d) { eo_row<T, TL, IDFT_IC(i)>(vp, vm, s, d); },
-                     [&](auto q, const pk& r) { t1[(ln * QP + IDFT_IC(q)) * NV + jv] = r; });
-    eo_out<N1, T, VC>([&](auto i, pk& s, pk& d) { eo_row<T, TM, IDFT_IC(i)>(vp, vm, s, d); },
-                      [&](auto a2, const pk& r) { ka[(ln * N1P + IDFT_IC(a2)) * NV + jv] = r; });
-    eo_out<N1, T, VC>([&](auto i, pk& s, pk& d) { eo_row<T, TD, IDFT_IC(i)>(vp, vm, s, d); },
-                      [&](auto a2, const pk& r) { kb[(ln * N1P + IDFT_IC(a2)) * NV + jv] = r; });
+    eo_split(v, keep_p[ki], keep_m[ki]);
+    eo_out<N1, T, VC>(
+        [&](auto i, pk& s, pk& d) { eo_row<T, TM, IDFT_IC(i)>(keep_p[ki], keep_m[ki], s, d); },
+        [&](auto a2, const pk& r) { rA[(ln * N1P + IDFT_IC(a2)) * NV + jv] = r; });
+    eo_out<N1, T, VC>(
+        [&](auto i, pk& s, pk& d) { eo_row<T, TD, IDFT_IC(i)>(keep_p[ki], keep_m[ki], s, d); },
+        [&](auto a2, const pk& r) { rB[(ln * N1P + IDFT_IC(a2)) * NV + jv] = r; });
   }
   __syncthreads();
-  // stage 2: y lines -- potential (c, qx), then kinetic (c, a), as two
-  // loops so that each holds only its own tables' constants
-  for (int it = tid; it < N1 * Q * NV; it += kThreads) {
-    const int jv = it % NV, ln = it / NV;
-    const int qx = ln % Q, c = ln / Q;
-    pk v[N1], vp[(N1 + 1) / 2], vm[N1 / 2];
-#pragma unroll
-    for (int b = 0; b < N1; ++b) v[b] = t1[((c * N1 + b) * QP + qx) * NV + jv];
-    eo_split(v, vp, vm);
-    eo_out<Q, T, VC>(
-        [&](auto i, pk& s, pk& d) { eo_row<T, TL, IDFT_IC(i)>(vp, vm, s, d); },
-        [&](auto qy, const pk& r) { t2[((c * Q + IDFT_IC(qy)) * Q + qx) * NV + jv] = r; });
-  }
+  // stage 2: y lines (c, a) of the kinetic chain, in place:
+  // k1 = M ka (over ka), k2 = D ka + M kb (over kb)
   for (int it = tid; it < N2 * NV; it += kThreads) {
     const int jv = it % NV, l2 = it / NV;
     const int a = l2 % N1, c = l2 / N1;
     pk va[N1], vb[N1], ap[(N1 + 1) / 2], am[N1 / 2], bp[(N1 + 1) / 2], bm[N1 / 2];
 #pragma unroll
     for (int b = 0; b < N1; ++b) {
-      va[b] = ka[((c * N1 + b) * N1P + a) * NV + jv];
-      vb[b] = kb[((c * N1 + b) * N1P + a) * NV + jv];
+      va[b] = rA[((c * N1 + b) * N1P + a) * NV + jv];
+      vb[b] = rB[((c * N1 + b) * N1P + a) * NV + jv];
     }
     eo_split(va, ap, am);
     eo_split(vb, bp, bm);
     eo_out<N1, T, VC>(
         [&](auto i, pk& s, pk& d) { eo_row<T, TM, IDFT_IC(i)>(ap, am, s, d); },
-        [&](auto b2, const pk& r) { k1[((c * N1 + IDFT_IC(b2)) * N1 + a) * NV + jv] = r; });
+        [&](auto b2, const pk& r) { rA[((c * N1 + IDFT_IC(b2)) * N1P + a) * NV + jv] = r; });
     eo_out<N1, T, VC>(
         [&](auto i, pk& s, pk& d) {
           eo_row<T, TD, IDFT_IC(i)>(ap, am, s, d);
           eo_row<T, TM, IDFT_IC(i)>(bp, bm, s, d);
         },
-        [&](auto b2, const pk& r) { k2[((c * N1 + IDFT_IC(b2)) * N1 + a) * NV + jv] = r; });
+        [&](auto b2, const pk& r) { rB[((c * N1 + IDFT_IC(b2)) * N1P + a) * NV + jv] = r; });
   }
   __syncthreads();
-  // stage 3: z lines -- potential (qy, qx) through the quadrature weights,
-  // then kinetic (b, a), again as two loops
-  for (int it = tid; it < Q2 * NV; it += kThreads) {
+  // stage 3: z lines (b, a) of the kinetic chain: yk = D k1 + M k2 (over k1)
+  for (int it = tid; it < N2 * NV; it += kThreads) {
+    const int jv = it % NV, l2 = it / NV;  // l2 = b * N1 + a
+    const int a = l2 % N1, b = l2 / N1;
+    pk v1[N1], v2[N1], p1[(N1 + 1) / 2], m1[N1 / 2], p2[(N1 + 1) / 2], m2[N1 / 2];
+#pragma unroll
+    for (int c = 0; c < N1; ++c) {
+      v1[c] = rA[((c * N1 + b) * N1P + a) * NV + jv];
+      v2[c] = rB[((c * N1 + b) * N1P + a) * NV + jv];
+    }
+    eo_split(v1, p1, m1);
+    eo_split(v2, p2, m2);
+    eo_out<N1, T, VC>(
+        [&](auto i, pk& s, pk& d) {
+          eo_row<T, TD, IDFT_IC(i)>(p1, m1, s, d);
+          eo_row<T, TM, IDFT_IC(i)>(p2, m2, s, d);
+        },
+        [&](auto c2, const pk& r) { rA[((IDFT_IC(c2) * N1 + b) * N1P + a) * NV + jv] = r; });
+  }
+  __syncthreads();
+  // stage 4: x lines of the potential chain from the kept input: t1 = Lq x (over k2)
+#pragma unroll
+  for (int ki = 0; ki < KI; ++ki) {
+    const int it = tid + ki * kThreads;
+    if (it >= N2 * NV) break;
     const int jv = it % NV, ln = it / NV;
+    eo_out<Q, T, VC>(
+        [&](auto i, pk& s, pk& d) { eo_row<T, TL, IDFT_IC(i)>(keep_p[ki], keep_m[ki], s, d); },
+        [&](auto q, const pk& r) { rB[(ln * QP + IDFT_IC(q)) * NV + jv] = r; });
+  }
+  __syncthreads();
+  // stage 5: y lines (c, qx): t2[c][qy][qx] = Lq t1
+  for (int it = tid; it < N1 * Q * NV; it += kThreads) {
+    const int jv = it % NV, ln = it / NV;
+    const int qx = ln % Q, c = ln / Q;
     pk v[N1], vp[(N1 + 1) / 2], vm[N1 / 2];
 #pragma unroll
-    for (int c = 0; c < N1; ++c) v[c] = t2[(c * Q2 + ln) * NV + jv];
+    for (int b = 0; b < N1; ++b) v[b] = rB[((c * N1 + b) * QP + qx) * NV + jv];
+    eo_split(v, vp, vm);
+    eo_out<Q, T, VC>(
+        [&](auto i, pk& s, pk& d) { eo_row<T, TL, IDFT_IC(i)>(vp, vm, s, d); },
+        [&](auto qy, const pk& r) { rD[((c * Q + IDFT_IC(qy)) * QP + qx) * NV + jv] = r; });
+  }
+  __syncthreads();
+  // stage 6: z lines (qy, qx) through the quadrature weights, in place:
+  // s2 = Lq^T (w .* (Lq t2)) (over t2)
+  for (int it = tid; it < Q2 * NV; it += kThreads) {
+    const int jv = it % NV, ln = it / NV;  // ln = qy * Q + qx
+    const int qx = ln % Q, qy = ln / Q;
+    pk v[N1], vp[(N1 + 1) / 2], vm[N1 / 2];
+#pragma unroll
+    for (int c = 0; c < N1; ++c) v[c] = rD[((c * Q + qy) * QP + qx) * NV + jv];
     eo_split(v, vp, vm);
     pk u[Q], up[(Q + 1) / 2], um[Q / 2];
     eo_out<Q, T, VC>([&](auto i, pk& s, pk& d) { eo_row<T, TL, IDFT_IC(i)>(vp, vm, s, d); },
@@ -852,52 +891,35 @@ __global__ __launch_bounds__(kThreads) void k_cell_sf(Geo g, T hq, const T* __re
                      });
     eo_split(u, up, um);
     eo_out<N1, T, VC>([&](auto i, pk& s, pk& d) { eo_row<T, TLt, IDFT_IC(i)>(up, um, s, d); },
-                      [&](auto c, const pk& r) { s2[(IDFT_IC(c) * Q2 + ln) * NV + jv] = r; });
-  }
-  for (int it = tid; it < N2 * NV; it += kThreads) {
-    const int jv = it % NV, l2 = it / NV;  // l2 = b * N1 + a
-    const int a = l2 % N1, b = l2 / N1;
-    pk v1[N1], v2[N1], p1[(N1 + 1) / 2], m1[N1 / 2], p2[(N1 + 1) / 2], m2[N1 / 2];
-#pragma unroll
-    for (int c = 0; c < N1; ++c) {
-      v1[c] = k1[(c * N2 + l2) * NV + jv];
-      v2[c] = k2[(c * N2 + l2) * NV + jv];
-    }
-    eo_split(v1, p1, m1);
-    eo_split(v2, p2, m2);
-    eo_out<N1, T, VC>(
-        [&](auto i, pk& s, pk& d) {
-          eo_row<T, TD, IDFT_IC(i)>(p1, m1, s, d);
-          eo_row<T, TM, IDFT_IC(i)>(p2, m2, s, d);
-        },
-        [&](auto c2, const pk& r) { yk[((IDFT_IC(c2) * N1 + b) * N1P + a) * NV + jv] = r; });
+                      [&](auto c, const pk& r) { rD[((IDFT_IC(c) * Q + qy) * QP + qx) * NV + jv] = r; });
   }
   __syncthreads();
-  // stage 4: y lines back (c, qx)
+  // stage 7: y lines back (c, qx), in place: s1[c][b][qx] = Lq^T s2 (over s2)
   for (int it = tid; it < N1 * Q * NV; it += kThreads) {
     const int jv = it % NV, ln = it / NV;
     const int qx = ln % Q, c = ln / Q;
     pk v[Q], vp[(Q + 1) / 2], vm[Q / 2];
 #pragma unroll
-    for (int qy = 0; qy < Q; ++qy) v[qy] = s2[((c * Q + qy) * Q + qx) * NV + jv];
+    for (int qy = 0; qy < Q; ++qy) v[qy] = rD[((c * Q + qy) * QP + qx) * NV + jv];
     eo_split(v, vp, vm);
     eo_out<N1, T, VC>(
         [&](auto i, pk& s, pk& d) { eo_row<T, TLt, IDFT_IC(i)>(vp, vm, s, d); },
-        [&](auto b, const pk& r) { s1[((c * N1 + IDFT_IC(b)) * QP + qx) * NV + jv] = r; });
+        [&](auto b, const pk& r) { rD[((c * Q + IDFT_IC(b)) * QP + qx) * NV + jv] = r; });
   }
   __syncthreads();
-  // stage 5: x lines back (c, b) plus the kinetic part, into the cell scratch
+  // stage 8: x lines back (c, b) plus the kinetic part, into the cell scratch
   for (int it = tid; it < N2 * NV; it += kThreads) {
     const int jv = it % NV, ln = it / NV;
     if (jv * VC >= bw) continue;
+    const int b = ln % N1, c = ln / N1;
     pk v[Q], vp[(Q + 1) / 2], vm[Q / 2];
 #pragma unroll
-    for (int qx = 0; qx < Q; ++qx) v[qx] = s1[(ln * QP + qx) * NV + jv];
+    for (int qx = 0; qx < Q; ++qx) v[qx] = rD[((c * Q + b) * QP + qx) * NV + jv];
     eo_split(v, vp, vm);
     T* yp = Yc + (size_t)((e * NE + ln * N1) * ub + c0 + jv * VC);
     eo_out<N1, T, VC>([&](auto i, pk& s, pk& d) { eo_row<T, TLt, IDFT_IC(i)>(vp, vm, s, d); },
                       [&](auto a, const pk& r) {
-                        const pk kin = yk[(ln * N1P + IDFT_IC(a)) * NV + jv];
+                        const pk kin = rA[(ln * N1P + IDFT_IC(a)) * NV + jv];
 #pragma unroll
                         for (int u = 0; u < VC; ++u)
                           if (jv * VC + u < bw) yp[(size_t)IDFT_IC(a) * ub + u] = hq * kin.v[u] + r.v[u];
