@@ -291,17 +291,18 @@ def kernels_only(lib, cfg, shape, reps):
           flush=True)
 
 
-def adjoint_only(lib, cfg, reps):
+def adjoint_only(lib, cfg, reps, graph=True):
     """The multivector adjoint solve alone (the reference's
     TestMultiVectorAdjointProblem path): FP64 eigensolve of the wells plus
     the seeded v_xc, then Jacobi-preconditioned MINRES on
     (H~ - eps_i) p_i = b_i for every column, to a residual of 1e-10 relative
-    to |b|_D, timed over `reps` solves, FP64 and mixed operator."""
+    to |b|_D, timed over `reps` solves, FP64 and mixed operator (graph=False:
+    the MINRES steps launched one by one instead of replayed as HIP graphs)."""
     from invdft_amd import abi, synthetic
     prob = synthetic.make_problem(lib, seed=1, target_tol=1e-10, **cfg)
     out = {}
     for name, prec in (("fp64", abi.FP64), ("mixed", abi.MIXED)):
-        ctx = prob.context(lib, precision=prec)
+        ctx = prob.context(lib, precision=prec, minres_graph=1 if graph else 0)
         ctx.set_potential(prob.vxc0)
         ev, _, _ = ctx.eigensolve(1e-10, 200)
         occ, fermi = ctx.occupancies()
@@ -424,7 +425,7 @@ def main():
         kernels_only(lib, cfg, args.shape, args.reps)
         return
     if args.adjoint_only:
-        adjoint_only(lib, cfg, args.reps)
+        adjoint_only(lib, cfg, args.reps, graph=not args.no_graph)
         return
     log(f"config {args.config}: {cfg}")
     t0 = time.perf_counter()

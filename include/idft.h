@@ -105,6 +105,11 @@ typedef struct idft_desc {
                                  the FP32 cell operator once every open
                                  column's residual estimate is below
                                  relax_ratio times the tolerance (0: never)    */
+  int32_t minres_fused;       /* HIP: with sum-factorised cell products, take
+                                 each MINRES step's alpha from the cell kernel
+                                 and run the Lanczos recurrence inside the node
+                                 gather (1), or store H~v and run it separately
+                                 (0); equal to rounding                        */
 } idft_desc;
 
 typedef struct idft_sizes {

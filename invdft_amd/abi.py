@@ -33,6 +33,7 @@ class Desc(C.Structure):
         ("rho_tol_constraints", C.c_double), ("gradient_mode", C.c_int32),
         ("lanczos_steps", C.c_int32), ("psi_checkpoint", C.c_int32), ("check_every", C.c_int32),
         ("seed", C.c_uint64), ("minres_graph", C.c_int32), ("relax_ratio", C.c_double),
+        ("minres_fused", C.c_int32),
     ]
 
 

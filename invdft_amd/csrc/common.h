@@ -132,7 +132,9 @@ constexpr int kMaxCols = 80;  // widest block the cell apply handles in one pass
 // Per-column MINRES state slots: st[slot * B + col].
 enum MinresSlot {
   MS_BETA, MS_OLDB, MS_DBAR, MS_EPSLN, MS_PHIBAR, MS_CS, MS_SN, MS_ALFA, MS_DONE, MS_IT, MS_TOL,
-  MS_BETA1, MS_CV, MS_CVN, MS_CR1, MS_CA2, MS_CWV, MS_CWW1, MS_CWW, MS_CX, MS_COUNT
+  MS_BETA1, MS_CV, MS_CVN, MS_CR1, MS_CA2, MS_CWV, MS_CWW1, MS_CWW, MS_CX,
+  MS_Q2, MS_Q3,  // D^2 r2.r2 and D r2.r1 of the current step (fused steps)
+  MS_COUNT
 };
 
 int64_t hamiltonian_tile_elems(int p);
@@ -277,18 +279,41 @@ void launch_force(const Geo& g, const DevTables& t, const double* Psi, const dou
                   hipStream_t s);
 void launch_zero_rows(int64_t N, int B, double* X, const unsigned char* mask, hipStream_t s);
 // MINRES
+// nq = 2: partial also holds (D r)^2 partials at row offset nblk (into Q2)
 void launch_minres_start(int B, int nblk, const double* partial, const double* negEig,
-                         double* st, hipStream_t s);
+                         double* st, hipStream_t s, int nq = 1);
 void launch_minres_alpha(int B, int nblk, const double* partial, double* st, hipStream_t s);
 template <typename T>
 int launch_minres_lanczos(int64_t N, int B, const T* y, const T* r2, T* rnew, const double* D,
                           const double* st, double* partial, hipStream_t s);
+// nq = 3: partial also holds the Q2 and Q3 partials at row offsets nblk, 2 nblk
 void launch_minres_givens(int B, int nblk, const double* partial, const double* negEig,
-                          double* st, hipStream_t s);
+                          double* st, hipStream_t s, int nq = 1);
 template <typename T>
 void launch_minres_update(int64_t N, int B, const T* r, const double* D, T* w1, const T* w, T* x,
                           const double* st, hipStream_t s);
 void launch_copy_done(int B, const double* st, int* flags, hipStream_t s);
+// Fused MINRES step for sum-factorised cell products (shape.kernel ==
+// CK_SUM_FACTOR), four launches per step instead of five and three block
+// streams fewer:
+//   start_dots   partials of r.D r and (D r)^2 of the right-hand side (2 x nblk
+//                rows), reduced by launch_minres_start(nq = 2) into beta and Q2
+//   cell_dot     cell product of v = cv D r2 (node scale D, column scale cv)
+//                with the per-cell dots x_e.(H_e x_e) (ncell x B)
+//   alpha_cell   alpha = sum of the cell dots + cv (cvn Q2 + cr1 Q3)
+//   gather       r1 <- S sum + (cvn D + ca2) r2 + cr1 r1 with the three next
+//                dots (3 x nblk rows), reduced by launch_minres_givens(nq = 3)
+int launch_minres_start_dots(int64_t N, int B, const double* r, const double* D, double* partial,
+                             hipStream_t s);
+template <typename T, typename TV>
+void launch_cell_dot(const Geo& g, const CellOp<T>& op, const T* Sg, const TV* X, int B,
+                     const double* xn, const double* xcol, T* Yc, double* cell_dots,
+                     const GemmShape& shape, hipStream_t s);
+void launch_minres_alpha_cell(int B, int ncell, const double* cell_dots, double* st, hipStream_t s);
+template <typename TY>
+int launch_minres_gather(const Geo& g, const TY* Yc, const double* S, const double* D,
+                         const double* r2, double* r1, int B, const double* st, double* partial,
+                         int gather_rows, hipStream_t s);
 
 // L-BFGS checkpoint settings of a context (idft_bfgs_set_checkpoint).
 struct BfgsCheckpoint {

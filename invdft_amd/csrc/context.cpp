@@ -106,7 +106,7 @@ struct idft_ctx {
   idft_stats st{};
   AdjointState adj;
   MinresWork<double> mw64;
-  DevBuf<double> mst, mx, mtmp, mnegc;
+  DevBuf<double> mst, mx, mtmp, mnegc, mcell;
   DevBuf<int> mflags, msel;
   idft_mv mvPsi, mvX;
 
@@ -856,9 +856,20 @@ struct idft_ctx {
       nblk = std::max(nblk, gather_blocks(g, bw, shape64.gather_rows, 8));
       if (canRelax) nblk = std::max(nblk, gather_blocks(g, bw, shape32.gather_rows, 8));
     }
-    partial.resize((size_t)nblk * b);
-    int nb = launch_coldot_partial<double>(N, b, r1, r1, Djac.p, partial.p, s);
-    launch_minres_start(b, nb, partial.p, adj.dnegEig.p, mst.p, s);
+    // Fused steps (sum-factorised cell products at every precision the solve
+    // may use): alpha comes from the cell kernel's per-cell dots, so the
+    // Lanczos recurrence runs inside the gather and y = H~ v is never stored.
+    const bool fused = d.minres_fused != 0 && shape64.kernel == CK_SUM_FACTOR &&
+                       (!canRelax || shape32.kernel == CK_SUM_FACTOR);
+    partial.resize((size_t)nblk * b * (fused ? 3 : 1));
+    if (fused) {
+      mcell.resize((size_t)t.ncell * b);
+      const int nb = launch_minres_start_dots(N, b, r1, Djac.p, partial.p, s);
+      launch_minres_start(b, nb, partial.p, adj.dnegEig.p, mst.p, s, 2);
+    } else {
+      const int nb = launch_coldot_partial<double>(N, b, r1, r1, Djac.p, partial.p, s);
+      launch_minres_start(b, nb, partial.p, adj.dnegEig.p, mst.p, s);
+    }
     // the compact block: bc columns, orig[k] the caller's column of column k
     int bc = b;
     std::vector<int> orig(b);
@@ -877,7 +888,37 @@ struct idft_ctx {
     // N*bc streams per step: 1 (cell product) + 3 (gather) + 3 (Lanczos) + 6
     // (update).  r2 = r_k (newest Lanczos residual), r1 = r_{k-1}; the step's
     // Lanczos vector v = cv D r2 is formed on the fly (not stored).
+    // Fused step: 4 launches and N*bc streams 1 (cell product) + 3 (gather:
+    // r2, r1 in, r1 out) + 6 (update).
+    auto fused_step = [&] {
+      const double* cv = mst.p + (size_t)MS_CV * bc;
+      if (f32op) {
+        st.applies_fp32++;
+        st.apply_cols_fp32 += bc;
+        launch_cell_dot<float, double>(g, op32(), S32.p, r2, bc, Djac.p, cv, Yc32.p, mcell.p,
+                                       shape32, s);
+      } else {
+        st.applies_fp64++;
+        st.apply_cols_fp64 += bc;
+        launch_cell_dot<double, double>(g, op64(), S.p, r2, bc, Djac.p, cv, Yc64.p, mcell.p,
+                                        shape64, s);
+      }
+      launch_minres_alpha_cell(bc, (int)t.ncell, mcell.p, mst.p, s);
+      const int nbk =
+          f32op ? launch_minres_gather<float>(g, Yc32.p, S.p, Djac.p, r2, r1, bc, mst.p, partial.p,
+                                              shape32.gather_rows, s)
+                : launch_minres_gather<double>(g, Yc64.p, S.p, Djac.p, r2, r1, bc, mst.p,
+                                               partial.p, shape64.gather_rows, s);
+      launch_minres_givens(bc, nbk, partial.p, mnegc.p, mst.p, s, 3);
+      launch_minres_update<double>(N, bc, r2, Djac.p, w1, w, xc, mst.p, s);
+      std::swap(r1, r2);
+      std::swap(w1, w);
+    };
     auto step = [&] {
+      if (fused) {
+        fused_step();
+        return;
+      }
       // y <- H~ v - eps v + cr1 r1, v = cv D r2;  partial dots of v.y  (one apply)
       ApplyEpilogue<double> ep;
       ep.Y = W.y.p; ep.alpha = 1.0;
@@ -1497,11 +1538,11 @@ idft_ctx* create_ctx(const idft_desc* d, const double* pot_base_q, const double*
     c->dt.Wc.upload(t.Wc.data(), t.Wc.size(), s);
     c->check_sf_tables();
     upload_sf_constants();
-    // default cell product: sum-factorised (no stored H_e) for blocks of up
-    // to 32 columns, where it is the faster kernel, if its cell scratch fits
-    // 32-bit indexing; the row pass over stored tiles otherwise
-    // (idft_autotune_apply refines both)
-    if (d->n_eig <= 32 && cell_scratch_elems(c->g, d->n_eig) < (int64_t)0xffffffffLL) {
+    // default cell product: sum-factorised (no stored H_e, 4 columns per
+    // work-group), the faster kernel at p = 4 and 5 for 12 to 80 columns,
+    // if its cell scratch fits 32-bit indexing; the row pass over stored
+    // tiles otherwise (idft_autotune_apply refines both)
+    if (cell_scratch_elems(c->g, d->n_eig) < (int64_t)0xffffffffLL) {
       for (GemmShape* sh : {&c->shape64, &c->shape32}) {
         sh->kernel = CK_SUM_FACTOR;
         sh->vectors_per_tile = 4;

@@ -253,7 +253,7 @@ void idft_desc_default(idft_desc* d) {
   d->frac_occ_tol = 1e-8; d->additional_states = 0; d->block_size = 0;
   d->rho_tol_constraints = 1e-6; d->gradient_mode = IDFT_GRADIENT_REFERENCE;
   d->lanczos_steps = 12; d->psi_checkpoint = 0; d->check_every = 8; d->seed = 1;
-  d->minres_graph = 1; d->relax_ratio = 1e3;
+  d->minres_graph = 1; d->relax_ratio = 1e3; d->minres_fused = 1;
 }
 
 int idft_geometry_sizes(const idft_desc* d, idft_sizes* s) {

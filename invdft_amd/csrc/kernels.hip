@@ -704,6 +704,20 @@ __device__ inline void eo_out(Rows&& rows, Put&& put) {
   });
 }
 #define IDFT_IC(x) decltype(x)::value
+// Entry A of a line from its even-odd split (eo_split): the inverse of the
+// split, to rounding.
+template <int C, int A, typename T, int VC>
+__device__ inline T eo_unsplit(const Pack<T, VC> (&xp)[(C + 1) / 2], const Pack<T, VC> (&xm)[C / 2],
+                               int u) {
+  if constexpr (2 * A + 1 == C) {
+    return xp[A].v[u];
+  } else if constexpr (A < C / 2) {
+    return T(0.5) * (xp[A].v[u] + xm[A].v[u]);
+  } else {
+    constexpr int J = C - 1 - A;
+    return T(0.5) * (xp[J].v[u] - xm[J].v[u]);
+  }
+}
 
 template <typename T, typename TV, int P, int CG, int VC, bool XN>
 __global__ __launch_bounds__(kThreads) void k_cell_sf(Geo g, T hq, const T* __restrict__ W,
@@ -711,7 +725,8 @@ __global__ __launch_bounds__(kThreads) void k_cell_sf(Geo g, T hq, const T* __re
                                                       const TV* __restrict__ X,
                                                       const double* __restrict__ xn,
                                                       const double* __restrict__ xcol,
-                                                      T* __restrict__ Yc, int B) {
+                                                      T* __restrict__ Yc, int B,
+                                                      double* __restrict__ dotp) {
   constexpr int N1 = P + 1, Q = P + 3, N2 = N1 * N1, NE = N2 * N1, Q2 = Q * Q;
   constexpr int QP = Q + 1, N1P = N1 + 1, NV = CG / VC;
   // x-line items per thread (lines x column packs over the work-group)
@@ -725,7 +740,7 @@ __global__ __launch_bounds__(kThreads) void k_cell_sf(Geo g, T hq, const T* __re
   //   A  [(c,b)][N1P]      ka -> k1 -> yk            (kinetic chain, in place)
   //   Bk [(c,b)][QP]       kb -> k2, then t1         (t1 once k2 is consumed)
   //   D  [c][Q][QP]        t2 -> s2 -> s1            (potential chain, in place)
-  __shared__ pk rA[N2 * N1P * NV];
+  __shared__ __align__(16) pk rA[N2 * N1P * NV];
   __shared__ pk rB[N2 * QP * NV];
   __shared__ pk rD[N1 * Q * QP * NV];
   // (cell, column group) pairs dealt to the 8 XCDs in contiguous runs, the
@@ -907,8 +922,16 @@ __global__ __launch_bounds__(kThreads) void k_cell_sf(Geo g, T hq, const T* __re
         [&](auto b, const pk& r) { rD[((c * Q + IDFT_IC(b)) * QP + qx) * NV + jv] = r; });
   }
   __syncthreads();
-  // stage 8: x lines back (c, b) plus the kinetic part, into the cell scratch
-  for (int it = tid; it < N2 * NV; it += kThreads) {
+  // stage 8: x lines back (c, b) plus the kinetic part, into the cell scratch;
+  // with dotp, also the cell's x.(H_e x) per column (x from the kept input
+  // lines: each item holds the same line as in stage 1)
+  double dotv[VC];
+#pragma unroll
+  for (int u = 0; u < VC; ++u) dotv[u] = 0.0;
+#pragma unroll
+  for (int ki = 0; ki < KI; ++ki) {
+    const int it = tid + ki * kThreads;
+    if (it >= N2 * NV) break;
     const int jv = it % NV, ln = it / NV;
     if (jv * VC >= bw) continue;
     const int b = ln % N1, c = ln / N1;
@@ -919,11 +942,44 @@ __global__ __launch_bounds__(kThreads) void k_cell_sf(Geo g, T hq, const T* __re
     T* yp = Yc + (size_t)((e * NE + ln * N1) * ub + c0 + jv * VC);
     eo_out<N1, T, VC>([&](auto i, pk& s, pk& d) { eo_row<T, TLt, IDFT_IC(i)>(vp, vm, s, d); },
                       [&](auto a, const pk& r) {
-                        const pk kin = rA[(ln * N1P + IDFT_IC(a)) * NV + jv];
+                        constexpr int A = IDFT_IC(a);
+                        const pk kin = rA[(ln * N1P + A) * NV + jv];
 #pragma unroll
-                        for (int u = 0; u < VC; ++u)
-                          if (jv * VC + u < bw) yp[(size_t)IDFT_IC(a) * ub + u] = hq * kin.v[u] + r.v[u];
+                        for (int u = 0; u < VC; ++u) {
+                          const T y = hq * kin.v[u] + r.v[u];
+                          if (jv * VC + u < bw) yp[(size_t)A * ub + u] = y;
+                          if (dotp)
+                            dotv[u] += (double)eo_unsplit<N1, A>(keep_p[ki], keep_m[ki], u) *
+                                       (double)y;
+                        }
                       });
+  }
+  if (dotp) {
+    // per column jv: the threads tid = jv (mod NV) (kThreads and 64 are
+    // multiples of NV), summed over each wave, then over the 4 waves
+    __syncthreads();  // stage 8 has read rA
+    double* red = reinterpret_cast<double*>(rA);
+    const int lane = tid & 63, wave = tid >> 6;
+#pragma unroll
+    for (int u = 0; u < VC; ++u) {
+      double t = dotv[u];
+#pragma unroll
+      for (int o = 32; o >= NV; o >>= 1) t += __shfl_xor(t, o, 64);
+      if (lane < NV) red[(wave * NV + lane) * VC + u] = t;
+    }
+    __syncthreads();
+    if (tid < NV) {
+#pragma unroll
+      for (int u = 0; u < VC; ++u) {
+        const int j = tid * VC + u;
+        if (j < bw) {
+          double t = 0.0;
+#pragma unroll
+          for (int w = 0; w < kThreads / 64; ++w) t += red[(w * NV + tid) * VC + u];
+          dotp[(size_t)e * ub + (uint32_t)(c0 + j)] = t;
+        }
+      }
+    }
   }
 }
 #undef IDFT_IC
@@ -984,6 +1040,64 @@ __device__ inline void vstore(T* __restrict__ p, const T (&o)[VW]) {
   }
 }
 
+// Sum of the (at most 8) cell contributions of node nidx for the VW columns
+// from j0: the cells sharing the node are found arithmetically on the
+// structured mesh (nodes n = (iz N1 + iy) N1 + ix).
+// The cells sharing node index i along one axis: (c0, a0) and, when the node
+// lies on an interior cell face (n = 2), (c1, a1), cell c and local index a,
+// in the order (left cell, right cell).
+struct AxisCells {
+  uint32_t c0, a0, c1, a1;
+  int n;
+};
+template <int P>
+__device__ inline AxisCells axis_cells(uint32_t i, uint32_t nc) {
+  const uint32_t q = i / P, r = i - q * P;
+  AxisCells A;
+  if (r != 0) { A.c0 = A.c1 = q; A.a0 = A.a1 = r; A.n = 1; }
+  else if (q == 0) { A.c0 = A.c1 = 0; A.a0 = A.a1 = 0; A.n = 1; }
+  else if (q == nc) { A.c0 = A.c1 = nc - 1; A.a0 = A.a1 = P; A.n = 1; }
+  else { A.c0 = q - 1; A.a0 = P; A.c1 = q; A.a1 = 0; A.n = 2; }
+  return A;
+}
+template <typename TY, int P, int VW>
+__device__ inline void gather_node(const Geo& g, FastDiv divN1, uint32_t B, uint32_t nidx,
+                                   const TY* __restrict__ Yc, uint32_t j0, double (&sum)[VW]) {
+  constexpr uint32_t n1 = P + 1, ne = n1 * n1 * n1;
+  const uint32_t N1 = g.N1, nc = g.nc;
+  const uint32_t row = fast_div(nidx, divN1), ix = nidx - row * N1;
+  const uint32_t iz = fast_div(row, divN1), iy = row - iz * N1;
+  // contributions (cell, local index) along each axis (selects, no arrays:
+  // run-time-indexed arrays would live in scratch memory)
+  const AxisCells X = axis_cells<P>(ix, nc), Y = axis_cells<P>(iy, nc), Z = axis_cells<P>(iz, nc);
+  // all 8 contribution slots are loaded before the first is summed, so
+  // the loads are in flight together
+  TY c8[8][VW];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int kz = k >> 2, ky = (k >> 1) & 1, kx = k & 1;
+    // no branch: a contribution the node does not have loads the node's
+    // first one (same cache line) and is then zeroed, so that all eight
+    // loads are issued before the first wait (behind a per-load branch
+    // the compiler waits for each load in turn)
+    const bool ok = kz < Z.n && ky < Y.n && kx < X.n;
+    const bool sz = ok && kz, sy = ok && ky, sx = ok && kx;
+    // the host checks cells n_e B < 2^32
+    const uint32_t e = ((sz ? Z.c1 : Z.c0) * nc + (sy ? Y.c1 : Y.c0)) * nc + (sx ? X.c1 : X.c0);
+    const uint32_t l = ((sz ? Z.a1 : Z.a0) * n1 + (sy ? Y.a1 : Y.a0)) * n1 + (sx ? X.a1 : X.a0);
+    TY t[VW];
+    vload<TY, VW>(Yc + ((e * ne + l) * B + j0), t);
+#pragma unroll
+    for (int v = 0; v < VW; ++v) c8[k][v] = ok ? t[v] : TY(0);
+  }
+#pragma unroll
+  for (int v = 0; v < VW; ++v) sum[v] = 0.0;
+#pragma unroll
+  for (int k = 0; k < 8; ++k)
+#pragma unroll
+    for (int v = 0; v < VW; ++v) sum[v] += (double)c8[k][v];
+}
+
 template <typename TY, typename T, int P, int VW>
 __global__ __launch_bounds__(kThreads) void k_cell_gather(
     Geo g, FastDiv divN1, uint32_t B, uint32_t R, const TY* __restrict__ Yc, const T* __restrict__ S,
@@ -992,9 +1106,8 @@ __global__ __launch_bounds__(kThreads) void k_cell_gather(
     const double* __restrict__ cb, const T* __restrict__ W, double c,
     const double* __restrict__ cc, const T* __restrict__ Xin, const double* __restrict__ xn,
     const double* __restrict__ xcol, T* __restrict__ V, double* __restrict__ partial) {
-  constexpr uint32_t n1 = P + 1, ne = n1 * n1 * n1;
   __shared__ double red[kThreads * VW];
-  const uint32_t N1 = g.N1, nc = g.nc;
+  const uint32_t N1 = g.N1;
   const uint32_t nb = B / VW;  // column groups of a node
   const uint32_t tid = threadIdx.x, per = kThreads / nb;
   const uint32_t jg = tid % nb, off = tid / nb, j0 = jg * VW;
@@ -1014,62 +1127,9 @@ __global__ __launch_bounds__(kThreads) void k_cell_gather(
     for (uint32_t rr = 0; rr < R; ++rr) {
       const uint32_t nidx = (blockIdx.x * R + rr) * per + off;
       if (nidx >= nn) break;
-      const uint32_t row = fast_div(nidx, divN1), ix = nidx - row * N1;
-      const uint32_t iz = fast_div(row, divN1), iy = row - iz * N1;
-      // contributions (cell, local index) along each axis
-      uint32_t cxs[2], axs[2];
-      int nx = 0;
-      {
-        const uint32_t q = ix / P, r = ix - q * P;
-        if (r == 0) {
-          if (q > 0) { cxs[nx] = q - 1; axs[nx++] = P; }
-          if (q < nc) { cxs[nx] = q; axs[nx++] = 0; }
-        } else { cxs[nx] = q; axs[nx++] = r; }
-      }
-      uint32_t cys[2], ays[2], czs[2], azs[2];
-      int ny = 0, nz = 0;
-      {
-        const uint32_t q = iy / P, r = iy - q * P;
-        if (r == 0) {
-          if (q > 0) { cys[ny] = q - 1; ays[ny++] = P; }
-          if (q < nc) { cys[ny] = q; ays[ny++] = 0; }
-        } else { cys[ny] = q; ays[ny++] = r; }
-      }
-      {
-        const uint32_t q = iz / P, r = iz - q * P;
-        if (r == 0) {
-          if (q > 0) { czs[nz] = q - 1; azs[nz++] = P; }
-          if (q < nc) { czs[nz] = q; azs[nz++] = 0; }
-        } else { czs[nz] = q; azs[nz++] = r; }
-      }
-      // all 8 contribution slots are loaded before the first is summed, so
-      // the loads are in flight together
-      TY c8[8][VW];
-#pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        const int kz = k >> 2, ky = (k >> 1) & 1, kx = k & 1;
-        // no branch: a contribution the node does not have loads the node's
-        // first one (same cache line) and is then zeroed, so that all eight
-        // loads are issued before the first wait (behind a per-load branch
-        // the compiler waits for each load in turn)
-        const bool ok = kz < nz && ky < ny && kx < nx;
-        const int kz2 = ok ? kz : 0, ky2 = ok ? ky : 0, kx2 = ok ? kx : 0;
-        // the host checks cells n_e B < 2^32
-        const uint32_t e = (czs[kz2] * nc + cys[ky2]) * nc + cxs[kx2];
-        const uint32_t l = (azs[kz2] * n1 + ays[ky2]) * n1 + axs[kx2];
-        TY t[VW];
-        vload<TY, VW>(Yc + ((e * ne + l) * B + j0), t);
-#pragma unroll
-        for (int v = 0; v < VW; ++v) c8[k][v] = ok ? t[v] : TY(0);
-      }
       double sum[VW];
-#pragma unroll
-      for (int v = 0; v < VW; ++v) sum[v] = 0.0;
-#pragma unroll
-      for (int k = 0; k < 8; ++k)
-#pragma unroll
-        for (int v = 0; v < VW; ++v) sum[v] += (double)c8[k][v];
-      const uint32_t n = row * N1 + ix;
+      gather_node<TY, P, VW>(g, divN1, B, nidx, Yc, j0, sum);
+      const uint32_t n = nidx;
       const uint32_t i = n * B + j0;
       const double sn = xn ? xn[n] : 1.0;
       const double sa = alpha * (double)S[n];
@@ -1111,6 +1171,79 @@ __global__ __launch_bounds__(kThreads) void k_cell_gather(
       partial[((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * B + tid] = t;
     }
   }
+}
+
+// MINRES step, phase 2 with the Lanczos recurrence fused in (sum-factorised
+// cell products, whose cell kernel has already produced v.Hv, so that the
+// step's alpha is known before this kernel runs):
+//   r_new = S sum + (cvn_j D r2 + ca2_j r2) + cr1_j r1     (over r1, in place)
+// with cvn = -eps/beta, ca2 = -alpha/beta, cr1 = -beta/beta_old (per column,
+// from the state), and block partials of the three node dots the next step
+// needs: r_new.D r_new (beta^2), D^2 r_new^2 and D r_new.r2, at partial +
+// k nblk B for k = 0, 1, 2.  Each thread reads and writes the same entries
+// of r1, so the update in place is safe.
+template <typename TY, typename T, int P, int VW>
+__global__ __launch_bounds__(kThreads) void k_minres_gather(
+    Geo g, FastDiv divN1, uint32_t B, uint32_t R, const TY* __restrict__ Yc,
+    const T* __restrict__ S, const double* __restrict__ D, const T* __restrict__ r2,
+    T* __restrict__ r1, const double* __restrict__ st, double* __restrict__ partial) {
+  __shared__ double red[kThreads * VW];
+  const uint32_t N1 = g.N1;
+  const uint32_t nb = B / VW;
+  const uint32_t tid = threadIdx.x, per = kThreads / nb;
+  const uint32_t jg = tid % nb, off = tid / nb, j0 = jg * VW;
+  double q1[VW], q2[VW], q3[VW];
+#pragma unroll
+  for (int v = 0; v < VW; ++v) q1[v] = q2[v] = q3[v] = 0.0;
+  if (off < per) {
+    double cvn[VW], ca2[VW], cr1[VW];
+#pragma unroll
+    for (int v = 0; v < VW; ++v) {
+      cvn[v] = st[MS_CVN * B + j0 + v];
+      ca2[v] = st[MS_CA2 * B + j0 + v];
+      cr1[v] = st[MS_CR1 * B + j0 + v];
+    }
+    const uint32_t nn = N1 * N1 * N1;
+    for (uint32_t rr = 0; rr < R; ++rr) {
+      const uint32_t n = (blockIdx.x * R + rr) * per + off;
+      if (n >= nn) break;
+      double sum[VW];
+      gather_node<TY, P, VW>(g, divN1, B, n, Yc, j0, sum);
+      const uint32_t i = n * B + j0;
+      const double dn = D[n];
+      const double sn = (double)S[n];
+      T xv[VW], zv[VW], ro[VW];
+      vload<T, VW>(r2 + i, xv);
+      vload<T, VW>(r1 + i, zv);
+#pragma unroll
+      for (int v = 0; v < VW; ++v) {
+        const double x = (double)xv[v];
+        const double t = sn * sum[v] + (cvn[v] * dn + ca2[v]) * x + cr1[v] * (double)zv[v];
+        ro[v] = (T)t;
+        const double r = (double)ro[v], dr = dn * r;
+        q1[v] += r * dr;
+        q2[v] += dr * dr;
+        q3[v] += dr * x;
+      }
+      vstore<T, VW>(r1 + i, ro);
+    }
+  }
+  const int64_t nblk = (int64_t)gridDim.x;
+  auto emit = [&](int k, const double (&q)[VW]) {
+#pragma unroll
+    for (int v = 0; v < VW; ++v) red[tid * VW + v] = q[v];
+    __syncthreads();
+    if (tid < B) {
+      const uint32_t g2 = tid / VW, v = tid % VW;
+      double t = 0.0;
+      for (uint32_t m = 0; m < per; ++m) t += red[(m * nb + g2) * VW + v];
+      partial[((int64_t)k * nblk + blockIdx.x) * B + tid] = t;
+    }
+    __syncthreads();
+  };
+  emit(0, q1);
+  emit(1, q2);
+  emit(2, q3);
 }
 
 // ------------------------------------------------------ Hamiltonian build
@@ -1694,11 +1827,15 @@ __global__ void k_zero_rows(uint32_t NB, uint32_t B, double* __restrict__ X,
 __global__ __launch_bounds__(kThreads) void k_minres_start(int B, int nblk,
                                                             const double* __restrict__ partial,
                                                             const double* __restrict__ negEig,
-                                                            double* __restrict__ st) {
+                                                            double* __restrict__ st, int nq) {
   __shared__ double red[kThreads];
   const int j = blockIdx.x;
   const double s = block_sum_col(partial, nblk, B, j, red);
+  __syncthreads();
+  const double q2 = nq >= 2 ? block_sum_col(partial + (int64_t)nblk * B, nblk, B, j, red) : 0.0;
   if (threadIdx.x != 0) return;
+  st[MS_Q2 * B + j] = q2;
+  st[MS_Q3 * B + j] = 0.0;
   const double beta = sqrt(fmax(s, 0.0));
   st[MS_BETA * B + j] = beta;
   st[MS_OLDB * B + j] = 0.0;
@@ -1726,6 +1863,56 @@ __global__ __launch_bounds__(kThreads) void k_minres_alpha(int B, int nblk,
   st[MS_ALFA * B + j] = s;
   const bool done = st[MS_DONE * B + j] != 0.0;
   st[MS_CA2 * B + j] = done ? 0.0 : -s / st[MS_BETA * B + j];
+}
+
+// alpha of a fused step: v.(H~ v) from the per-cell dots x_e.(H_e x_e)
+// (x = S v the cell product's input) plus the node terms of the step's
+// output, v.(cvn D r2 + cr1 r1) = cv (cvn Q2 + cr1 Q3) with v = cv D r2.
+__global__ __launch_bounds__(kThreads) void k_minres_alpha_cell(int B, int ncell,
+                                                                 const double* __restrict__ cd,
+                                                                 double* __restrict__ st) {
+  __shared__ double red[kThreads];
+  const int j = blockIdx.x;
+  const double s = block_sum_col(cd, ncell, B, j, red);
+  if (threadIdx.x != 0) return;
+  const double cv = st[MS_CV * B + j];
+  const double a = s + cv * (st[MS_CVN * B + j] * st[MS_Q2 * B + j] +
+                             st[MS_CR1 * B + j] * st[MS_Q3 * B + j]);
+  st[MS_ALFA * B + j] = a;
+  const bool done = st[MS_DONE * B + j] != 0.0;
+  st[MS_CA2 * B + j] = done ? 0.0 : -a / st[MS_BETA * B + j];
+}
+
+// Partials of r.D r and (D r).(D r) per column (rows [0, nblk) and
+// [nblk, 2 nblk) of partial): the start of a fused MINRES solve.
+__global__ __launch_bounds__(kThreads) void k_minres_start_dots(int64_t N, int B,
+                                                                const double* __restrict__ r,
+                                                                const double* __restrict__ D,
+                                                                double* __restrict__ partial) {
+  __shared__ double red[2][kThreads];
+  const int tid = threadIdx.x;
+  const int per = kThreads / B;
+  const int col = tid % B, off = tid / B;
+  double s1 = 0.0, s2 = 0.0;
+  if (off < per) {
+    for (int64_t n = (int64_t)blockIdx.x * per + off; n < N; n += (int64_t)gridDim.x * per) {
+      const double v = r[n * B + col], dv = D[n] * v;
+      s1 += v * dv;
+      s2 += dv * dv;
+    }
+  }
+  red[0][tid] = s1;
+  red[1][tid] = s2;
+  __syncthreads();
+  if (tid < B) {
+    double t1 = 0.0, t2 = 0.0;
+    for (int k = 0; k < per; ++k) {
+      t1 += red[0][k * B + tid];
+      t2 += red[1][k * B + tid];
+    }
+    partial[(int64_t)blockIdx.x * B + tid] = t1;
+    partial[((int64_t)gridDim.x + blockIdx.x) * B + tid] = t2;
+  }
 }
 
 // r_new = y + ca2 r2 (stored in rnew), partial dot(r_new, D r_new).  The
@@ -1766,12 +1953,21 @@ __global__ __launch_bounds__(kThreads) void k_minres_lanczos(int64_t N, int B,
 __global__ __launch_bounds__(kThreads) void k_minres_givens(int B, int nblk,
                                                              const double* __restrict__ partial,
                                                              const double* __restrict__ negEig,
-                                                             double* __restrict__ st) {
+                                                             double* __restrict__ st, int nq) {
   __shared__ double red[kThreads];
   const int j = blockIdx.x;
   const double s = block_sum_col(partial, nblk, B, j, red);
+  double q2 = 0.0, q3 = 0.0;
+  if (nq >= 3) {
+    __syncthreads();
+    q2 = block_sum_col(partial + (int64_t)nblk * B, nblk, B, j, red);
+    __syncthreads();
+    q3 = block_sum_col(partial + (int64_t)2 * nblk * B, nblk, B, j, red);
+  }
   if (threadIdx.x != 0) return;
   double* S = st;
+  S[MS_Q2 * B + j] = q2;
+  S[MS_Q3 * B + j] = q3;
   if (S[MS_DONE * B + j] != 0.0) {
     S[MS_CWV * B + j] = 0.0; S[MS_CWW1 * B + j] = 0.0; S[MS_CWW * B + j] = 0.0;
     S[MS_CX * B + j] = 0.0; S[MS_CV * B + j] = 0.0; S[MS_CR1 * B + j] = 0.0;
@@ -1945,7 +2141,8 @@ bool sf_vc_supported(int cols, int vc, bool fp64) {
 
 template <typename T, typename TV, int P, int CG>
 static void cell_sf_cg(const Geo& g, const CellOp<T>& op, const T* S, const TV* X,
-                       const double* xn, const double* xcol, T* Yc, int B, hipStream_t s) {
+                       const double* xn, const double* xcol, T* Yc, int B, double* dotp,
+                       hipStream_t s) {
   check_u32(g.N * B);
   check_u32(cell_scratch_elems(g, B));
   const int64_t npair = g.ncell * ((B + CG - 1) / CG);
@@ -1956,26 +2153,28 @@ static void cell_sf_cg(const Geo& g, const CellOp<T>& op, const T* S, const TV* 
   // carry a node scale
   if constexpr (std::is_same<TV, double>::value) {
     if (xn) {
-      k_cell_sf<T, TV, P, CG, 1, true><<<grid, block, 0, s>>>(g, hq, op.W, S, X, xn, xcol, Yc, B);
+      k_cell_sf<T, TV, P, CG, 1, true><<<grid, block, 0, s>>>(g, hq, op.W, S, X, xn, xcol, Yc, B,
+                                                               dotp);
       return;
     }
   } else {
     if (xn) throw Error(IDFT_EINVAL, "node-scaled input needs FP64 vectors");
   }
-  k_cell_sf<T, TV, P, CG, 1, false><<<grid, block, 0, s>>>(g, hq, op.W, S, X, nullptr, xcol, Yc, B);
+  k_cell_sf<T, TV, P, CG, 1, false><<<grid, block, 0, s>>>(g, hq, op.W, S, X, nullptr, xcol, Yc, B,
+                                                             dotp);
 }
 
 template <typename T, typename TV, int P>
 static void cell_sf_p(const Geo& g, const CellOp<T>& op, const T* S, const TV* X,
                       const double* xn, const double* xcol, T* Yc, int B, int cg, int vc,
-                      hipStream_t s) {
+                      double* dotp, hipStream_t s) {
   if (!op.W) throw Error(IDFT_ESTATE, "quadrature weights of the cell operator not set");
   if (!sf_vc_supported(cg, vc, std::is_same<T, double>::value))
     throw Error(IDFT_EINVAL, "shape not supported by the sum-factorised kernel");
   switch (cg) {
-    case 2: cell_sf_cg<T, TV, P, 2>(g, op, S, X, xn, xcol, Yc, B, s); break;
-    case 4: cell_sf_cg<T, TV, P, 4>(g, op, S, X, xn, xcol, Yc, B, s); break;
-    case 8: cell_sf_cg<T, TV, P, 8>(g, op, S, X, xn, xcol, Yc, B, s); break;
+    case 2: cell_sf_cg<T, TV, P, 2>(g, op, S, X, xn, xcol, Yc, B, dotp, s); break;
+    case 4: cell_sf_cg<T, TV, P, 4>(g, op, S, X, xn, xcol, Yc, B, dotp, s); break;
+    case 8: cell_sf_cg<T, TV, P, 8>(g, op, S, X, xn, xcol, Yc, B, dotp, s); break;
     default: throw Error(IDFT_EINVAL, "columns per work-group not supported by the sum-factorised kernel");
   }
 }
@@ -1983,12 +2182,13 @@ static void cell_sf_p(const Geo& g, const CellOp<T>& op, const T* S, const TV* X
 template <typename T, typename TV, int P>
 static void cell_gemm_p(const Geo& g, const CellOp<T>& op, const T* S, const TV* X,
                         const double* xn, const double* xcol, T* Yc, int B,
-                        const GemmShape& shape, hipStream_t s) {
+                        const GemmShape& shape, hipStream_t s, double* dotp = nullptr) {
   if (shape.kernel == CK_SUM_FACTOR) {
     cell_sf_p<T, TV, P>(g, op, S, X, xn, xcol, Yc, B, shape.vectors_per_tile,
-                        shape.cells_per_wg, s);
+                        shape.cells_per_wg, dotp, s);
     return;
   }
+  if (dotp) throw Error(IDFT_EINVAL, "per-cell dots need the sum-factorised cell product");
   const T* H = op.H;
   if (!H) throw Error(IDFT_ESTATE, "stored cell matrices not built");
   const int vt = std::min(shape.vectors_per_tile, ((B + 15) / 16) * 16);
@@ -2139,6 +2339,66 @@ template int launch_cell_apply<float, double>(const Geo&, const CellOp<float>&, 
                                               const double*, const double*, float*, int,
                                               const ApplyEpilogue<double>&, const GemmShape&,
                                               hipStream_t);
+
+template <typename T, typename TV>
+void launch_cell_dot(const Geo& g, const CellOp<T>& op, const T* Sg, const TV* X, int B,
+                     const double* xn, const double* xcol, T* Yc, double* cell_dots,
+                     const GemmShape& shape, hipStream_t s) {
+  if (B < 1 || B > kMaxCols) throw Error(IDFT_EINVAL, "block width out of range");
+  if (shape.kernel != CK_SUM_FACTOR || !cell_dots)
+    throw Error(IDFT_EINVAL, "per-cell dots need the sum-factorised cell product");
+  switch (g.p) {
+    case 2: cell_gemm_p<T, TV, 2>(g, op, Sg, X, xn, xcol, Yc, B, shape, s, cell_dots); break;
+    case 3: cell_gemm_p<T, TV, 3>(g, op, Sg, X, xn, xcol, Yc, B, shape, s, cell_dots); break;
+    case 4: cell_gemm_p<T, TV, 4>(g, op, Sg, X, xn, xcol, Yc, B, shape, s, cell_dots); break;
+    case 5: cell_gemm_p<T, TV, 5>(g, op, Sg, X, xn, xcol, Yc, B, shape, s, cell_dots); break;
+    default: throw Error(IDFT_EINVAL, "degree not supported by the HIP kernels (2..5)");
+  }
+  IDFT_HIP(hipGetLastError());
+}
+template void launch_cell_dot<double, double>(const Geo&, const CellOp<double>&, const double*,
+                                              const double*, int, const double*, const double*,
+                                              double*, double*, const GemmShape&, hipStream_t);
+template void launch_cell_dot<float, double>(const Geo&, const CellOp<float>&, const float*,
+                                             const double*, int, const double*, const double*,
+                                             float*, double*, const GemmShape&, hipStream_t);
+
+template <typename TY>
+int launch_minres_gather(const Geo& g, const TY* Yc, const double* S, const double* D,
+                         const double* r2, double* r1, int B, const double* st, double* partial,
+                         int gather_rows, hipStream_t s) {
+  if (B < 1 || B > kMaxCols) throw Error(IDFT_EINVAL, "block width out of range");
+  check_u32(g.N * B);
+  check_u32(cell_scratch_elems(g, B));
+  const FastDiv divN1 = make_fastdiv((uint32_t)g.N1);
+  unsigned gx, gy;
+  uint32_t R;
+  const int vw = gather_vw(B, (int)sizeof(double));
+  gather_grid(g, B, vw, gather_rows, gx, gy, R);
+#define IDFT_MG(PP, VW)                                                                      \
+  k_minres_gather<TY, double, PP, VW><<<dim3(gx), kThreads, 0, s>>>(g, divN1, (uint32_t)B, R, Yc, \
+                                                                   S, D, r2, r1, st, partial)
+#define IDFT_MG_P(PP)   \
+  if (vw == 2) IDFT_MG(PP, 2); \
+  else IDFT_MG(PP, 1);
+  switch (g.p) {
+    case 2: IDFT_MG_P(2); break;
+    case 3: IDFT_MG_P(3); break;
+    case 4: IDFT_MG_P(4); break;
+    case 5: IDFT_MG_P(5); break;
+    default: throw Error(IDFT_EINVAL, "degree not supported by the HIP kernels (2..5)");
+  }
+#undef IDFT_MG_P
+#undef IDFT_MG
+  IDFT_HIP(hipGetLastError());
+  return (int)gx;
+}
+template int launch_minres_gather<double>(const Geo&, const double*, const double*, const double*,
+                                          const double*, double*, int, const double*, double*, int,
+                                          hipStream_t);
+template int launch_minres_gather<float>(const Geo&, const float*, const double*, const double*,
+                                         const double*, double*, int, const double*, double*, int,
+                                         hipStream_t);
 
 template <typename T>
 void launch_lincomb(int64_t N, int B, T* Y, double beta, const T* X, double a, const double* ca,
@@ -2339,8 +2599,20 @@ void launch_zero_rows(int64_t N, int B, double* X, const unsigned char* mask, hi
 
 
 void launch_minres_start(int B, int nblk, const double* partial, const double* negEig, double* st,
-                         hipStream_t s) {
-  k_minres_start<<<B, kThreads, 0, s>>>(B, nblk, partial, negEig, st);
+                         hipStream_t s, int nq) {
+  k_minres_start<<<B, kThreads, 0, s>>>(B, nblk, partial, negEig, st, nq);
+  IDFT_HIP(hipGetLastError());
+}
+int launch_minres_start_dots(int64_t N, int B, const double* r, const double* D, double* partial,
+                             hipStream_t s) {
+  if (B > kThreads) throw Error(IDFT_EINVAL, "block too wide for column reductions");
+  const int nb = coldot_blocks(N, B);
+  k_minres_start_dots<<<nb, kThreads, 0, s>>>(N, B, r, D, partial);
+  IDFT_HIP(hipGetLastError());
+  return nb;
+}
+void launch_minres_alpha_cell(int B, int ncell, const double* cell_dots, double* st, hipStream_t s) {
+  k_minres_alpha_cell<<<B, kThreads, 0, s>>>(B, ncell, cell_dots, st);
   IDFT_HIP(hipGetLastError());
 }
 void launch_minres_alpha(int B, int nblk, const double* partial, double* st, hipStream_t s) {
@@ -2360,8 +2632,8 @@ template int launch_minres_lanczos<float>(int64_t, int, const float*, const floa
 template int launch_minres_lanczos<double>(int64_t, int, const double*, const double*, double*,
                                            const double*, const double*, double*, hipStream_t);
 void launch_minres_givens(int B, int nblk, const double* partial, const double* negEig, double* st,
-                          hipStream_t s) {
-  k_minres_givens<<<B, kThreads, 0, s>>>(B, nblk, partial, negEig, st);
+                          hipStream_t s, int nq) {
+  k_minres_givens<<<B, kThreads, 0, s>>>(B, nblk, partial, negEig, st, nq);
   IDFT_HIP(hipGetLastError());
 }
 template <typename T>

@@ -1216,6 +1216,7 @@ void idft_desc_default(idft_desc* d) {
   d->lanczos_steps = 12; d->psi_checkpoint = 0; d->check_every = 8; d->seed = 1;
   d->minres_graph = 1;  // HIP launch option; no effect here
   d->relax_ratio = 1e3;  // mixed-precision option; no effect here
+  d->minres_fused = 1;   // HIP launch option; no effect here
 }
 
 static bool desc_ok(const idft_desc* d) {

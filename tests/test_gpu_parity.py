@@ -167,6 +167,55 @@ def test_minres_graph_replay_matches_launches(glib, olib, prec):
         assert np.array_equal(out[0][k][0], out[1][k][0])
 
 
+@pytest.mark.parametrize("prec", [abi.FP64, abi.MIXED])
+def test_minres_fused_step_matches_separate(glib, olib, prec):
+    """Fused MINRES steps (alpha from the sum-factorised cell kernel's per-cell
+    dots, Lanczos recurrence inside the node gather) against the separate
+    Lanczos kernel (minres_fused = 0) and against the stored-H row pass,
+    which always runs the separate steps: the same solutions to rounding and
+    iteration counts within 2, and the oracle's solution."""
+    _say("minres_fused", prec)
+    prob = small_problem(olib)
+    o = prob.context(olib)
+    o.set_potential(prob.vxc0)
+    ev, _, _ = o.eigensolve(1e-11, 60)
+    occ, fermi = o.occupancies()
+    psi = o.eigenvectors()
+    u = prob.rho_target_q - 0.5 * o.density()
+    B = psi.shape[1]
+    pairs = [v for i in range(B) for j in range(B) if abs(ev[i] - ev[j]) < 0.002 for v in (i, j)]
+    mpsi_o, _ = _adjoint_setup(o, prob, psi, occ, ev.copy(), u, pairs, fermi)
+    xo = o.mv(B)
+    o.adj_compute_rhs(None, xo, o.mv(B))
+    so = o.mv(B)
+    its_o = np.array(o.minres_solve(so, None, 1e-12, 3000, distribute=False))
+    ref = so.get()
+    out = {}
+    for name, fused, row_pass in (("fused", 1, False), ("separate", 0, False),
+                                  ("row_pass", 1, True)):
+        g = prob.context(glib, precision=prec, minres_fused=fused)
+        if row_pass:
+            g.set_apply_shape(abi.FP64, 1, 16, 0, 0)
+            if prec == abi.MIXED:
+                g.set_apply_shape(abi.MIXED, 1, 16, 0, 0)
+        g.set_potential(prob.vxc0)
+        mpsi, _ = _adjoint_setup(g, prob, psi, occ, ev.copy(), u, pairs, fermi)
+        x = g.mv(B)
+        g.adj_compute_rhs(None, x, g.mv(B))
+        sol = g.mv(B)
+        its = np.array(g.minres_solve(sol, None, 1e-12, 3000, distribute=False))
+        out[name] = (sol.get(), its)
+        print(f"  {name}: iterations {its.tolist()}, rel to oracle {rel(out[name][0], ref):.2e}",
+              flush=True)
+        del mpsi
+    print(f"  oracle iterations {its_o.tolist()}", flush=True)
+    for name in ("separate", "row_pass"):
+        assert rel(out["fused"][0], out[name][0]) < 1e-8
+        assert np.all(np.abs(out["fused"][1] - out[name][1]) <= 2)
+    assert rel(out["fused"][0], ref) < 1e-8
+    assert np.all(np.abs(out["fused"][1] - its_o) <= 3)
+
+
 @pytest.mark.parametrize("p", [4, 5])
 @pytest.mark.parametrize("mode", [abi.GRADIENT_REFERENCE, abi.GRADIENT_EXACT])
 def test_get_force(glib, olib, mode, p):
