@@ -294,15 +294,19 @@ void launch_minres_update(int64_t N, int B, const T* r, const double* D, T* w1, 
                           const double* st, hipStream_t s);
 void launch_copy_done(int B, const double* st, int* flags, hipStream_t s);
 // Fused MINRES step for sum-factorised cell products (shape.kernel ==
-// CK_SUM_FACTOR), four launches per step instead of five and three block
-// streams fewer:
+// CK_SUM_FACTOR), four launches per step instead of five and four block
+// streams fewer (9 instead of 13, plus one update launch per run of steps):
 //   start_dots   partials of r.D r and (D r)^2 of the right-hand side (2 x nblk
 //                rows), reduced by launch_minres_start(nq = 2) into beta and Q2
 //   cell_dot     cell product of v = cv D r2 (node scale D, column scale cv)
 //                with the per-cell dots x_e.(H_e x_e) (ncell x B)
 //   alpha_cell   alpha = sum of the cell dots + cv (cvn Q2 + cr1 Q3)
 //   gather       r1 <- S sum + (cvn D + ca2) r2 + cr1 r1 with the three next
-//                dots (3 x nblk rows), reduced by launch_minres_givens(nq = 3)
+//                dots (3 x nblk rows), reduced by launch_minres_givens(nq = 3);
+//                with w != nullptr also the previous step's update
+//                w <- cwv D r1 + cww1 w + cww w1, x += cx w (read before r1 is
+//                overwritten), so that launch_minres_update runs only after the
+//                last step of a run of steps
 int launch_minres_start_dots(int64_t N, int B, const double* r, const double* D, double* partial,
                              hipStream_t s);
 template <typename T, typename TV>
@@ -313,7 +317,8 @@ void launch_minres_alpha_cell(int B, int ncell, const double* cell_dots, double*
 template <typename TY>
 int launch_minres_gather(const Geo& g, const TY* Yc, const double* S, const double* D,
                          const double* r2, double* r1, int B, const double* st, double* partial,
-                         int gather_rows, hipStream_t s);
+                         int gather_rows, hipStream_t s, double* w = nullptr,
+                         const double* w1 = nullptr, double* x = nullptr);
 
 // L-BFGS checkpoint settings of a context (idft_bfgs_set_checkpoint).
 struct BfgsCheckpoint {

@@ -889,7 +889,10 @@ struct idft_ctx {
     // (update).  r2 = r_k (newest Lanczos residual), r1 = r_{k-1}; the step's
     // Lanczos vector v = cv D r2 is formed on the fly (not stored).
     // Fused step: 4 launches and N*bc streams 1 (cell product) + 3 (gather:
-    // r2, r1 in, r1 out) + 6 (update).
+    // r2, r1 in, r1 out) + 5 (the previous step's update inside the gather:
+    // w, w1 and x in, w and x out; r1 is read anyway).  A run of steps ends
+    // with finish_run(), the last step's update on its own (6 streams).
+    bool pending = false;  // a step whose update has not run yet
     auto fused_step = [&] {
       const double* cv = mst.p + (size_t)MS_CV * bc;
       if (f32op) {
@@ -904,15 +907,27 @@ struct idft_ctx {
                                         shape64, s);
       }
       launch_minres_alpha_cell(bc, (int)t.ncell, mcell.p, mst.p, s);
+      // the previous step's update (w <- its new direction, x) if pending:
+      // after that step's swaps w holds its oldest direction and w1 the newer
+      double* wu = pending ? w : nullptr;
+      const double* w1u = pending ? w1 : nullptr;
+      double* xu = pending ? xc : nullptr;
       const int nbk =
           f32op ? launch_minres_gather<float>(g, Yc32.p, S.p, Djac.p, r2, r1, bc, mst.p, partial.p,
-                                              shape32.gather_rows, s)
+                                              shape32.gather_rows, s, wu, w1u, xu)
                 : launch_minres_gather<double>(g, Yc64.p, S.p, Djac.p, r2, r1, bc, mst.p,
-                                               partial.p, shape64.gather_rows, s);
+                                               partial.p, shape64.gather_rows, s, wu, w1u, xu);
       launch_minres_givens(bc, nbk, partial.p, mnegc.p, mst.p, s, 3);
-      launch_minres_update<double>(N, bc, r2, Djac.p, w1, w, xc, mst.p, s);
+      pending = true;
       std::swap(r1, r2);
       std::swap(w1, w);
+    };
+    // the last step's update of a run of fused steps (r1 = its Lanczos
+    // residual after the swap, w its pending direction, w1 the newer one)
+    auto finish_run = [&] {
+      if (!pending) return;
+      launch_minres_update<double>(N, bc, r1, Djac.p, w, w1, xc, mst.p, s);
+      pending = false;
     };
     auto step = [&] {
       if (fused) {
@@ -1030,6 +1045,7 @@ struct idft_ctx {
           IDFT_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
           try {
             for (int i = 0; i < n; ++i) step();
+            finish_run();
           } catch (...) {
             (void)hipStreamEndCapture(s, &graph);
             if (graph) (void)hipGraphDestroy(graph);
@@ -1050,6 +1066,7 @@ struct idft_ctx {
         IDFT_HIP(hipGraphLaunch(exec, s));
       } else {
         for (int i = 0; i < n; ++i) step();
+        finish_run();
       }
       k += n;
     }

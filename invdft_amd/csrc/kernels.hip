@@ -512,8 +512,9 @@ __global__ __launch_bounds__(kSymmThreads) void k_cell_symm(Geo g, const T* __re
 // One work-group handles one cell and CG columns; an item is one line of the
 // cell's tensor along one axis for VC consecutive columns (VC-wide LDS
 // accesses).  Intermediates live in LDS as [entry][CG / VC] packs, with padded
-// line strides (N1+1, Q+1 entries) so that the 64 lanes of a wave spread over
-// the banks.  The kinetic chain runs first and the x-line input stays in
+// line strides (N1 and Q rounded up to odd: 5, 7 at p = 4; 7, 9 at p = 5) so
+// that the 64 lanes of a wave spread over the banks (an even stride such as 6
+// or 8 at p = 4 folds the lines of a wave onto a few banks).  The kinetic chain runs first and the x-line input stays in
 // registers for the potential chain, and the y and z stages overwrite their
 // input in place (each item reads its whole line before writing the same
 // positions), so only three regions are live:
@@ -728,7 +729,7 @@ __global__ __launch_bounds__(kThreads) void k_cell_sf(Geo g, T hq, const T* __re
                                                       T* __restrict__ Yc, int B,
                                                       double* __restrict__ dotp) {
   constexpr int N1 = P + 1, Q = P + 3, N2 = N1 * N1, NE = N2 * N1, Q2 = Q * Q;
-  constexpr int QP = Q + 1, N1P = N1 + 1, NV = CG / VC;
+  constexpr int QP = Q | 1, N1P = N1 | 1, NV = CG / VC;
   // x-line items per thread (lines x column packs over the work-group)
   constexpr int KI = (N2 * NV + kThreads - 1) / kThreads;
   typedef Pack<T, VC> pk;
@@ -923,8 +924,9 @@ __global__ __launch_bounds__(kThreads) void k_cell_sf(Geo g, T hq, const T* __re
   }
   __syncthreads();
   // stage 8: x lines back (c, b) plus the kinetic part, into the cell scratch;
-  // with dotp, also the cell's x.(H_e x) per column (x from the kept input
-  // lines: each item holds the same line as in stage 1)
+  // with dotp (node-scaled MINRES inputs only, XN), also the cell's x.(H_e x)
+  // per column (x from the kept input lines: each item holds the same line as
+  // in stage 1)
   double dotv[VC];
 #pragma unroll
   for (int u = 0; u < VC; ++u) dotv[u] = 0.0;
@@ -948,13 +950,15 @@ __global__ __launch_bounds__(kThreads) void k_cell_sf(Geo g, T hq, const T* __re
                         for (int u = 0; u < VC; ++u) {
                           const T y = hq * kin.v[u] + r.v[u];
                           if (jv * VC + u < bw) yp[(size_t)A * ub + u] = y;
-                          if (dotp)
-                            dotv[u] += (double)eo_unsplit<N1, A>(keep_p[ki], keep_m[ki], u) *
-                                       (double)y;
+                          if constexpr (XN) {
+                            if (dotp)
+                              dotv[u] += (double)eo_unsplit<N1, A>(keep_p[ki], keep_m[ki], u) *
+                                         (double)y;
+                          }
                         }
                       });
   }
-  if (dotp) {
+  if (XN && dotp) {
     // per column jv: the threads tid = jv (mod NV) (kThreads and 64 are
     // multiples of NV), summed over each wave, then over the 4 waves
     __syncthreads();  // stage 8 has read rA
@@ -1182,11 +1186,17 @@ __global__ __launch_bounds__(kThreads) void k_cell_gather(
 // needs: r_new.D r_new (beta^2), D^2 r_new^2 and D r_new.r2, at partial +
 // k nblk B for k = 0, 1, 2.  Each thread reads and writes the same entries
 // of r1, so the update in place is safe.
-template <typename TY, typename T, int P, int VW>
+// UPD: also the previous step's direction and solution update, deferred to
+// here because this kernel reads r1 (that step's Lanczos residual) anyway:
+//   w <- cwv D r1 + cww1 w + cww w1,  x += cx w   (the previous step's
+// coefficients, which the state still holds: its Givens kernel wrote them
+// and this step's has not run yet)
+template <typename TY, typename T, int P, int VW, bool UPD>
 __global__ __launch_bounds__(kThreads) void k_minres_gather(
     Geo g, FastDiv divN1, uint32_t B, uint32_t R, const TY* __restrict__ Yc,
     const T* __restrict__ S, const double* __restrict__ D, const T* __restrict__ r2,
-    T* __restrict__ r1, const double* __restrict__ st, double* __restrict__ partial) {
+    T* __restrict__ r1, const double* __restrict__ st, double* __restrict__ partial,
+    T* __restrict__ w, const T* __restrict__ w1, T* __restrict__ xs) {
   __shared__ double red[kThreads * VW];
   const uint32_t N1 = g.N1;
   const uint32_t nb = B / VW;
@@ -1196,12 +1206,18 @@ __global__ __launch_bounds__(kThreads) void k_minres_gather(
 #pragma unroll
   for (int v = 0; v < VW; ++v) q1[v] = q2[v] = q3[v] = 0.0;
   if (off < per) {
-    double cvn[VW], ca2[VW], cr1[VW];
+    double cvn[VW], ca2[VW], cr1[VW], cwv[VW], cw1[VW], cww[VW], cx[VW];
 #pragma unroll
     for (int v = 0; v < VW; ++v) {
       cvn[v] = st[MS_CVN * B + j0 + v];
       ca2[v] = st[MS_CA2 * B + j0 + v];
       cr1[v] = st[MS_CR1 * B + j0 + v];
+      if constexpr (UPD) {
+        cwv[v] = st[MS_CWV * B + j0 + v];
+        cw1[v] = st[MS_CWW1 * B + j0 + v];
+        cww[v] = st[MS_CWW * B + j0 + v];
+        cx[v] = st[MS_CX * B + j0 + v];
+      }
     }
     const uint32_t nn = N1 * N1 * N1;
     for (uint32_t rr = 0; rr < R; ++rr) {
@@ -1215,6 +1231,12 @@ __global__ __launch_bounds__(kThreads) void k_minres_gather(
       T xv[VW], zv[VW], ro[VW];
       vload<T, VW>(r2 + i, xv);
       vload<T, VW>(r1 + i, zv);
+      T wv[VW], w1v[VW], sv[VW];
+      if constexpr (UPD) {
+        vload<T, VW>(w + i, wv);
+        vload<T, VW>(w1 + i, w1v);
+        vload<T, VW>(xs + i, sv);
+      }
 #pragma unroll
       for (int v = 0; v < VW; ++v) {
         const double x = (double)xv[v];
@@ -1224,8 +1246,18 @@ __global__ __launch_bounds__(kThreads) void k_minres_gather(
         q1[v] += r * dr;
         q2[v] += dr * dr;
         q3[v] += dr * x;
+        if constexpr (UPD) {
+          const double wn = cwv[v] * (dn * (double)zv[v]) + cw1[v] * (double)wv[v] +
+                            cww[v] * (double)w1v[v];
+          wv[v] = (T)wn;
+          sv[v] = (T)((double)sv[v] + cx[v] * wn);
+        }
       }
       vstore<T, VW>(r1 + i, ro);
+      if constexpr (UPD) {
+        vstore<T, VW>(w + i, wv);
+        vstore<T, VW>(xs + i, sv);
+      }
     }
   }
   const int64_t nblk = (int64_t)gridDim.x;
@@ -2130,8 +2162,8 @@ void launch_build_hamiltonian(const Geo& g, const DevTables& t, double hh, const
 }
 
 // Columns per work-group (CG) and per item (VC) of the sum-factorised
-// kernel built into the library: CG in {2, 4, 8} (its LDS holds about 1644 CG
-// values at p = 5, so FP64 stops at 8), one column per item.  The kernel is
+// kernel built into the library: CG in {2, 4, 8} (its LDS holds 1008 CG
+// values at p = 5, 64.5 KB in FP64 at CG = 8), one column per item.  The kernel is
 // written for VC columns per item; wider items and 16-column groups measured
 // slower on every mesh tried and are not instantiated.
 bool sf_cols_supported(int cols, bool) { return cols == 2 || cols == 4 || cols == 8; }
@@ -2149,6 +2181,8 @@ static void cell_sf_cg(const Geo& g, const CellOp<T>& op, const T* S, const TV* 
   check_u32(8 * ((npair + 7) / 8));
   dim3 grid((unsigned)(8 * ((npair + 7) / 8))), block(kThreads);
   const T hq = (T)op.kin_scale;
+  // per-cell dots only with node-scaled inputs (the MINRES steps)
+  if (dotp && !xn) throw Error(IDFT_EINVAL, "per-cell dots need a node-scaled input");
   // node-scaled inputs (MINRES) come as FP64 vectors; FP32 vectors never
   // carry a node scale
   if constexpr (std::is_same<TV, double>::value) {
@@ -2161,7 +2195,7 @@ static void cell_sf_cg(const Geo& g, const CellOp<T>& op, const T* S, const TV* 
     if (xn) throw Error(IDFT_EINVAL, "node-scaled input needs FP64 vectors");
   }
   k_cell_sf<T, TV, P, CG, 1, false><<<grid, block, 0, s>>>(g, hq, op.W, S, X, nullptr, xcol, Yc, B,
-                                                             dotp);
+                                                             nullptr);
 }
 
 template <typename T, typename TV, int P>
@@ -2366,7 +2400,7 @@ template void launch_cell_dot<float, double>(const Geo&, const CellOp<float>&, c
 template <typename TY>
 int launch_minres_gather(const Geo& g, const TY* Yc, const double* S, const double* D,
                          const double* r2, double* r1, int B, const double* st, double* partial,
-                         int gather_rows, hipStream_t s) {
+                         int gather_rows, hipStream_t s, double* w, const double* w1, double* x) {
   if (B < 1 || B > kMaxCols) throw Error(IDFT_EINVAL, "block width out of range");
   check_u32(g.N * B);
   check_u32(cell_scratch_elems(g, B));
@@ -2375,9 +2409,15 @@ int launch_minres_gather(const Geo& g, const TY* Yc, const double* S, const doub
   uint32_t R;
   const int vw = gather_vw(B, (int)sizeof(double));
   gather_grid(g, B, vw, gather_rows, gx, gy, R);
-#define IDFT_MG(PP, VW)                                                                      \
-  k_minres_gather<TY, double, PP, VW><<<dim3(gx), kThreads, 0, s>>>(g, divN1, (uint32_t)B, R, Yc, \
-                                                                   S, D, r2, r1, st, partial)
+#define IDFT_MG(PP, VW)                                                                   \
+  do {                                                                                     \
+    if (w)                                                                                 \
+      k_minres_gather<TY, double, PP, VW, true><<<dim3(gx), kThreads, 0, s>>>(             \
+          g, divN1, (uint32_t)B, R, Yc, S, D, r2, r1, st, partial, w, w1, x);              \
+    else                                                                                   \
+      k_minres_gather<TY, double, PP, VW, false><<<dim3(gx), kThreads, 0, s>>>(            \
+          g, divN1, (uint32_t)B, R, Yc, S, D, r2, r1, st, partial, nullptr, nullptr, nullptr); \
+  } while (0)
 #define IDFT_MG_P(PP)   \
   if (vw == 2) IDFT_MG(PP, 2); \
   else IDFT_MG(PP, 1);
@@ -2395,10 +2435,10 @@ int launch_minres_gather(const Geo& g, const TY* Yc, const double* S, const doub
 }
 template int launch_minres_gather<double>(const Geo&, const double*, const double*, const double*,
                                           const double*, double*, int, const double*, double*, int,
-                                          hipStream_t);
+                                          hipStream_t, double*, const double*, double*);
 template int launch_minres_gather<float>(const Geo&, const float*, const double*, const double*,
                                          const double*, double*, int, const double*, double*, int,
-                                         hipStream_t);
+                                         hipStream_t, double*, const double*, double*);
 
 template <typename T>
 void launch_lincomb(int64_t N, int B, T* Y, double beta, const T* X, double a, const double* ca,
