@@ -210,10 +210,26 @@ def c3_leg(lib, cheb_tol):
         ctx.close()
     fm, ff = forces["mixed"], forces["fp64"]
     err = [float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in zip(fm, ff)]
+    # roofline of this workload's FP64 cell product (the FP64 run's tuned
+    # shape), measured live by idft_profile_apply as for the headline kernel
+    roof = None
+    sh = out["fp64"]["tile_shapes"]["fp64"]
+    if sh["kernel"] == 2:
+        p, nc = prob.desc.degree, prob.desc.n_cells
+        cells, N = nc ** 3, (nc * p + 1) ** 3
+        flops = cells * B * sf_flops_per_cell_column(p)
+        nbytes = 8 * (cells * (p + 3) ** 3 + N * (B + 1) + cells * (p + 1) ** 3 * B)
+        ms = out["fp64"]["fp64_cell_product_ms"]
+        ach = flops / (ms * 1e-3) / 1e12
+        roof = {"bound": "valu", "kernel": f"k_cell_sf<double,double,{p},{sh['vectors_per_tile']},1,false>",
+                "achieved": round(ach, 2), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(ach / FP64_PEAK_TFLOPS, 4), "ms_per_launch": ms,
+                "flops_per_launch": flops, "bytes_per_launch": nbytes,
+                "achieved_GBs": round(nbytes / (ms * 1e-3) / 1e9, 1)}
     return {"workload": "batched_40c_p4", "nodes": int(prob.desc.n_cells * prob.desc.degree + 1) ** 3,
             "orbitals": B, "evals_per_iter": 2, "mixed": out["mixed"], "fp64": out["fp64"],
             "speedup_mixed_vs_fp64": round(out["fp64"]["s_per_iter"] / out["mixed"]["s_per_iter"], 3),
-            "force_rel_err_mixed_vs_fp64": err}
+            "force_rel_err_mixed_vs_fp64": err, "roofline": roof}
 
 
 def c4_large(lib, evals):

@@ -514,20 +514,23 @@ __global__ __launch_bounds__(kSymmThreads) void k_cell_symm(Geo g, const T* __re
 // accesses).  Intermediates live in LDS as [entry][CG / VC] packs, with padded
 // line strides (N1 and Q rounded up to odd: 5, 7 at p = 4; 7, 9 at p = 5) so
 // that the 64 lanes of a wave spread over the banks (an even stride such as 6
-// or 8 at p = 4 folds the lines of a wave onto a few banks).  The kinetic chain runs first and the x-line input stays in
-// registers for the potential chain, and the y and z stages overwrite their
-// input in place (each item reads its whole line before writing the same
-// positions), so only three regions are live:
+// or 8 at p = 4 folds the lines of a wave onto a few banks).  The kinetic
+// chain runs first and the x-line input stays in registers for the potential
+// chain, and the y and z stages overwrite their input in place (each item
+// reads its whole line before writing the same positions; in stage 5 the
+// item (c, qx) owns the column [c][*][qx] of D, which holds t1 along b and
+// then t2 along qy).  Region Bk is dead once stage 3 has run, so D shares its
+// memory:
 //   1  x lines (c,b):  ka = M x -> A,  kb = D x -> Bk      (x gathered from HBM)
 //   2  y lines (c,a):  k1 = M ka -> A,  k2 = D ka + M kb -> Bk
 //   3  z lines (b,a):  yk = D k1 + M k2 -> A
-//   4  x lines (c,b):  t1 = Lq x -> Bk                     (x from registers)
+//   4  x lines (c,b):  t1[c][b][qx] = Lq x -> D             (x from registers)
 //   5  y lines (c,qx): t2[c][qy][qx] = Lq t1 -> D
 //   6  z lines (qy,qx): s2 = Lq^T (w .* (Lq t2)) -> D
 //   7  y lines (c,qx): s1[c][b][qx] = Lq^T s2 -> D
 //   8  x lines (c,b):  Yc[e][(c,b,a)][j] = (Lq^T s1)[a] + (h/4) yk[a]
-// At p = 5 that is 1008 entries per column (52.6 KB -> 32 KB per work-group
-// of 4 FP64 columns: 5 work-groups per CU instead of 3).
+// At p = 5 that is 252 + 432 = 684 entries per column: 21.9 KB per work-group
+// of 4 FP64 columns, 7 work-groups (28 waves) per CU.
 __host__ __device__ constexpr int cmax(int a, int b) { return a > b ? a : b; }
 
 template <typename T, int VC>
@@ -737,13 +740,14 @@ __global__ __launch_bounds__(kThreads) void k_cell_sf(Geo g, T hq, const T* __re
   typedef SfTab<P, 1> TLt;
   typedef SfTab<P, 2> TM;
   typedef SfTab<P, 3> TD;
-  // three regions, each [entry][NV] packs:
+  // regions, each [entry][NV] packs:
   //   A  [(c,b)][N1P]      ka -> k1 -> yk            (kinetic chain, in place)
-  //   Bk [(c,b)][QP]       kb -> k2, then t1         (t1 once k2 is consumed)
-  //   D  [c][Q][QP]        t2 -> s2 -> s1            (potential chain, in place)
+  //   Bk [(c,b)][N1P]      kb -> k2                  (stages 1-3)
+  //   D  [c][Q][QP]        t1 -> t2 -> s2 -> s1      (stages 4-8, over Bk)
   __shared__ __align__(16) pk rA[N2 * N1P * NV];
-  __shared__ pk rB[N2 * QP * NV];
-  __shared__ pk rD[N1 * Q * QP * NV];
+  __shared__ pk rBD[cmax(N2 * N1P, N1 * Q * QP) * NV];
+  pk* const rB = rBD;
+  pk* const rD = rBD;
   // (cell, column group) pairs dealt to the 8 XCDs in contiguous runs, the
   // groups of one cell at consecutive slots of one XCD, so that they share
   // that XCD's L2 for the cell's rows of X and of the cell scratch (each group
@@ -865,24 +869,26 @@ __global__ __launch_bounds__(kThreads) void k_cell_sf(Geo g, T hq, const T* __re
         [&](auto c2, const pk& r) { rA[((IDFT_IC(c2) * N1 + b) * N1P + a) * NV + jv] = r; });
   }
   __syncthreads();
-  // stage 4: x lines of the potential chain from the kept input: t1 = Lq x (over k2)
+  // stage 4: x lines of the potential chain from the kept input:
+  // t1[c][b][qx] = Lq x (over k2, which stage 3 has consumed)
 #pragma unroll
   for (int ki = 0; ki < KI; ++ki) {
     const int it = tid + ki * kThreads;
     if (it >= N2 * NV) break;
     const int jv = it % NV, ln = it / NV;
+    const int b = ln % N1, c = ln / N1;
     eo_out<Q, T, VC>(
         [&](auto i, pk& s, pk& d) { eo_row<T, TL, IDFT_IC(i)>(keep_p[ki], keep_m[ki], s, d); },
-        [&](auto q, const pk& r) { rB[(ln * QP + IDFT_IC(q)) * NV + jv] = r; });
+        [&](auto q, const pk& r) { rD[((c * Q + b) * QP + IDFT_IC(q)) * NV + jv] = r; });
   }
   __syncthreads();
-  // stage 5: y lines (c, qx): t2[c][qy][qx] = Lq t1
+  // stage 5: y lines (c, qx): t2[c][qy][qx] = Lq t1, in place (over t1)
   for (int it = tid; it < N1 * Q * NV; it += kThreads) {
     const int jv = it % NV, ln = it / NV;
     const int qx = ln % Q, c = ln / Q;
     pk v[N1], vp[(N1 + 1) / 2], vm[N1 / 2];
 #pragma unroll
-    for (int b = 0; b < N1; ++b) v[b] = rB[((c * N1 + b) * QP + qx) * NV + jv];
+    for (int b = 0; b < N1; ++b) v[b] = rD[((c * Q + b) * QP + qx) * NV + jv];
     eo_split(v, vp, vm);
     eo_out<Q, T, VC>(
         [&](auto i, pk& s, pk& d) { eo_row<T, TL, IDFT_IC(i)>(vp, vm, s, d); },
@@ -2162,8 +2168,8 @@ void launch_build_hamiltonian(const Geo& g, const DevTables& t, double hh, const
 }
 
 // Columns per work-group (CG) and per item (VC) of the sum-factorised
-// kernel built into the library: CG in {2, 4, 8} (its LDS holds 1008 CG
-// values at p = 5, 64.5 KB in FP64 at CG = 8), one column per item.  The kernel is
+// kernel built into the library: CG in {2, 4, 8} (its LDS holds 684 CG
+// values at p = 5, 43.8 KB in FP64 at CG = 8), one column per item.  The kernel is
 // written for VC columns per item; wider items and 16-column groups measured
 // slower on every mesh tried and are not instantiated.
 bool sf_cols_supported(int cols, bool) { return cols == 2 || cols == 4 || cols == 8; }
