@@ -83,6 +83,25 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def fma_probe(lib, chains=16):
+    """TFLOP/s of an FMA loop with `chains` independent chains per thread,
+    FP64 and unpacked FP32 (idft_fma_probe)."""
+    import ctypes as C
+    from invdft_amd import abi
+    fn = lib.idft_fma_probe
+    fn.restype = C.c_int
+    fn.argtypes = [C.c_int32, C.c_int32, C.POINTER(C.c_double)]
+    out = {}
+    for name, prec in (("fp64", abi.FP64), ("fp32", abi.MIXED)):
+        v = C.c_double()
+        rc = fn(prec, chains, C.byref(v))
+        if rc != 0:
+            raise RuntimeError(f"idft_fma_probe failed ({rc})")
+        out[name] = round(v.value, 2)
+    log(f"attainable FMA rate ({chains} chains per thread): {out} TFLOP/s")
+    return out
+
+
 def run_bfgs(lib, prob, precision, warmup, steps, tol_kw):
     from invdft_amd import abi
     ctx = prob.context(lib, precision=precision, **tol_kw)
@@ -533,6 +552,13 @@ def main():
     roof = roofline(mixed["tuned"]["fp64"], False, ms_cell64, ms_op64)
     ms_op32, ms_cell32 = cm.profile_apply(B, abi.MIXED, reps=20)
     extra = {"fp32_cell_product": roofline(mixed["tuned"]["fp32"], True, ms_cell32, ms_op32)}
+    # attainable FMA rates of this GPU (16 independent chains per thread):
+    # the rate a latency-limited kernel can approach, beside the published
+    # peaks that `frac` uses
+    fma = fma_probe(lib)
+    extra["attainable_fma_tflops"] = fma
+    if roof.get("unit") == "TFLOP/s" and fma.get("fp64"):
+        roof["attainable_fma"] = fma["fp64"]
     if cf is not None:
         m64, g64 = cf.profile_apply(B, abi.FP64, reps=20)
         extra["fp64_run_cell_product"] = {"ms_per_launch": round(g64, 4),

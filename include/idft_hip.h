@@ -90,6 +90,14 @@ int idft_adj_vmult_noalloc(idft_ctx* ctx, idft_mv* Ax, idft_mv* x, int32_t block
 /* Device pointer of a multivector handle (owned or wrapped). */
 double* idft_mv_device_ptr(const idft_mv* mv);
 
+/* Attainable FMA rate of device 0 (the current device): TFLOP/s of a kernel
+ * whose threads run `chains` (4, 8 or 16) independent multiply-add chains,
+ * FP64 (precision = IDFT_FP64) or unpacked FP32 (IDFT_MIXED), 32 work-groups
+ * of 256 threads per CU.  No counterpart in the reference; the benchmark
+ * reports it beside the published peaks, as the rate a latency-limited
+ * kernel can reach.  Returns IDFT_EINVAL for other chain counts. */
+int idft_fma_probe(int32_t precision, int32_t chains, double* tflops);
+
 /* Device memory held by this library's allocations in the process: current
  * bytes and the peak since the last reset (reset_peak != 0 restarts the
  * peak at the current value after reading).  No counterpart in the

@@ -161,6 +161,9 @@ struct ApplyEpilogue {
   double* dot_partial = nullptr;
 };
 constexpr int kGatherBlocks = 16384;
+// grid cap of the fused MINRES gather (its work-groups loop over node chunks),
+// which bounds its partial rows per step to 3 x B x 4096
+constexpr int kMinresGatherBlocks = 4096;
 // rows of the gather's dot partials for vectors of vbytes-byte elements
 int gather_blocks(const Geo& g, int B, int rows, int vbytes);
 // Cell-product kernels.
@@ -293,13 +296,16 @@ template <typename T>
 void launch_minres_update(int64_t N, int B, const T* r, const double* D, T* w1, const T* w, T* x,
                           const double* st, hipStream_t s);
 void launch_copy_done(int B, const double* st, int* flags, hipStream_t s);
+// TFLOP/s of `chains` independent FMA chains per thread over 32 work-groups
+// per CU (FP64 or FP32, unpacked), timed with HIP events on stream s
+double launch_fma_probe(bool fp64, int chains, hipStream_t s);
 // Fused MINRES step for sum-factorised cell products (shape.kernel ==
 // CK_SUM_FACTOR), four launches per step instead of five and four block
 // streams fewer (9 instead of 13, plus one update launch per run of steps):
 //   start_dots   partials of r.D r and (D r)^2 of the right-hand side (2 x nblk
 //                rows), reduced by launch_minres_start(nq = 2) into beta and Q2
 //   cell_dot     cell product of v = cv D r2 (node scale D, column scale cv)
-//                with the per-cell dots x_e.(H_e x_e) (ncell x B)
+//                with the per-cell dots x_e.(H_e x_e) (column-major, B x ncell)
 //   alpha_cell   alpha = sum of the cell dots + cv (cvn Q2 + cr1 Q3)
 //   gather       r1 <- S sum + (cvn D + ca2) r2 + cr1 r1 with the three next
 //                dots (3 x nblk rows), reduced by launch_minres_givens(nq = 3);

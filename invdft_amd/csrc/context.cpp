@@ -1015,16 +1015,29 @@ struct idft_ctx {
       for (int j = 0; j < bc; ++j)
         if (!flags[j]) open.push_back(j);
       if (open.empty() || k >= maxit) break;
-      if ((int)open.size() < bc && bc - (int)open.size() >= std::max(1, bc / 16)) {
-        compact(open);
+      // repack once enough columns have finished, keeping the width even
+      // (one finished column rides along, inert: its coefficients are zero)
+      // so that the block kernels move two columns per 16-byte access
+      std::vector<int> keep = open;
+      if (keep.size() % 2 == 1 && (int)keep.size() < bc) {
+        for (int j = 0; j < bc; ++j)
+          if (flags[j]) { keep.push_back(j); break; }
+        std::sort(keep.begin(), keep.end());
+      }
+      if ((int)keep.size() < bc && bc - (int)open.size() >= std::max(1, bc / 16)) {
+        std::vector<int> kflags(keep.size());
+        std::vector<double> pb(keep.size());
+        for (size_t i = 0; i < keep.size(); ++i) {
+          kflags[i] = flags[keep[i]];
+          pb[i] = phibar[keep[i]];
+        }
+        compact(keep);
         st.minres_compactions++;
         if (verbose)
           std::fprintf(stderr, "[idft] minres: step %d, block compacted to %d columns\n", k, bc);
-        // the compacted flags and estimates are the open columns' values
-        std::vector<double> pb(bc);
-        for (int i = 0; i < bc; ++i) pb[i] = phibar[open[i]];
+        // the compacted flags and estimates are the kept columns' values
         phibar = pb;
-        flags.assign(bc, 0);
+        flags = kflags;
       }
       if (canRelax && !f32op) {
         bool ready = true;
@@ -1850,6 +1863,21 @@ int idft_mv_wrap(idft_ctx* c, double* dptr, int32_t ncols, idft_mv** out) {
 }
 
 double* idft_mv_device_ptr(const idft_mv* v) { return v ? v->p() : nullptr; }
+
+int idft_fma_probe(int32_t precision, int32_t chains, double* tflops) {
+  if (!tflops || (precision != IDFT_FP64 && precision != IDFT_MIXED) ||
+      (chains != 4 && chains != 8 && chains != 16))
+    return IDFT_EINVAL;
+  try {
+    *tflops = launch_fma_probe(precision == IDFT_FP64, chains, nullptr);
+    IDFT_HIP(hipDeviceSynchronize());
+  } catch (const Error& e) {
+    return e.code;
+  } catch (...) {
+    return IDFT_EHIP;
+  }
+  return IDFT_OK;
+}
 
 int idft_device_memory(int64_t* cur, int64_t* peak, int32_t reset) {
   if (cur) *cur = devmem_current().load();

@@ -40,6 +40,20 @@ __device__ inline double block_sum_col(const double* __restrict__ partial, int n
   return red[0];
 }
 
+// Sum of n contiguous values by one work-group (all threads call it).
+__device__ inline double block_sum_contig(const double* __restrict__ v, int n, double* red) {
+  const int tid = threadIdx.x;
+  double s = 0.0;
+  for (int b = tid; b < n; b += kThreads) s += v[b];
+  red[tid] = s;
+  __syncthreads();
+  for (int st = kThreads / 2; st > 0; st >>= 1) {
+    if (tid < st) red[tid] += red[tid + st];
+    __syncthreads();
+  }
+  return red[0];
+}
+
 inline void check_u32(int64_t n) {
   if (n >= (int64_t)0xffffffffLL) throw Error(IDFT_EINVAL, "multivector too large for 32-bit indexing");
 }
@@ -931,8 +945,8 @@ __global__ __launch_bounds__(kThreads) void k_cell_sf(Geo g, T hq, const T* __re
   __syncthreads();
   // stage 8: x lines back (c, b) plus the kinetic part, into the cell scratch;
   // with dotp (node-scaled MINRES inputs only, XN), also the cell's x.(H_e x)
-  // per column (x from the kept input lines: each item holds the same line as
-  // in stage 1)
+  // per column, into dotp[column * ncell + cell] (x from the kept input lines:
+  // each item holds the same line as in stage 1)
   double dotv[VC];
 #pragma unroll
   for (int u = 0; u < VC; ++u) dotv[u] = 0.0;
@@ -986,7 +1000,7 @@ __global__ __launch_bounds__(kThreads) void k_cell_sf(Geo g, T hq, const T* __re
           double t = 0.0;
 #pragma unroll
           for (int w = 0; w < kThreads / 64; ++w) t += red[(w * NV + tid) * VC + u];
-          dotp[(size_t)e * ub + (uint32_t)(c0 + j)] = t;
+          dotp[(size_t)(uint32_t)(c0 + j) * ncell + e] = t;
         }
       }
     }
@@ -1189,8 +1203,8 @@ __global__ __launch_bounds__(kThreads) void k_cell_gather(
 //   r_new = S sum + (cvn_j D r2 + ca2_j r2) + cr1_j r1     (over r1, in place)
 // with cvn = -eps/beta, ca2 = -alpha/beta, cr1 = -beta/beta_old (per column,
 // from the state), and block partials of the three node dots the next step
-// needs: r_new.D r_new (beta^2), D^2 r_new^2 and D r_new.r2, at partial +
-// k nblk B for k = 0, 1, 2.  Each thread reads and writes the same entries
+// needs: r_new.D r_new (beta^2), D^2 r_new^2 and D r_new.r2, at
+// partial[(k B + j) nblk + block] for k = 0, 1, 2.  Each thread reads and writes the same entries
 // of r1, so the update in place is safe.
 // UPD: also the previous step's direction and solution update, deferred to
 // here because this kernel reads r1 (that step's Lanczos residual) anyway:
@@ -1226,8 +1240,12 @@ __global__ __launch_bounds__(kThreads) void k_minres_gather(
       }
     }
     const uint32_t nn = N1 * N1 * N1;
+    // groups of R consecutive chunks, strided over a grid of at most
+    // kMinresGatherBlocks work-groups (fewer partial rows to reduce)
+    const uint32_t ngroups = (nn + R * per - 1) / (R * per);
+    for (uint32_t grp = blockIdx.x; grp < ngroups; grp += gridDim.x)
     for (uint32_t rr = 0; rr < R; ++rr) {
-      const uint32_t n = (blockIdx.x * R + rr) * per + off;
+      const uint32_t n = (grp * R + rr) * per + off;
       if (n >= nn) break;
       double sum[VW];
       gather_node<TY, P, VW>(g, divN1, B, n, Yc, j0, sum);
@@ -1266,6 +1284,8 @@ __global__ __launch_bounds__(kThreads) void k_minres_gather(
       }
     }
   }
+  // partials column-major, [k][column][block], so that each column's
+  // reduction reads contiguous memory
   const int64_t nblk = (int64_t)gridDim.x;
   auto emit = [&](int k, const double (&q)[VW]) {
 #pragma unroll
@@ -1275,7 +1295,7 @@ __global__ __launch_bounds__(kThreads) void k_minres_gather(
       const uint32_t g2 = tid / VW, v = tid % VW;
       double t = 0.0;
       for (uint32_t m = 0; m < per; ++m) t += red[(m * nb + g2) * VW + v];
-      partial[((int64_t)k * nblk + blockIdx.x) * B + tid] = t;
+      partial[((int64_t)k * B + tid) * nblk + blockIdx.x] = t;
     }
     __syncthreads();
   };
@@ -1911,7 +1931,7 @@ __global__ __launch_bounds__(kThreads) void k_minres_alpha_cell(int B, int ncell
                                                                  double* __restrict__ st) {
   __shared__ double red[kThreads];
   const int j = blockIdx.x;
-  const double s = block_sum_col(cd, ncell, B, j, red);
+  const double s = block_sum_contig(cd + (int64_t)j * ncell, ncell, red);
   if (threadIdx.x != 0) return;
   const double cv = st[MS_CV * B + j];
   const double a = s + cv * (st[MS_CVN * B + j] * st[MS_Q2 * B + j] +
@@ -1994,13 +2014,15 @@ __global__ __launch_bounds__(kThreads) void k_minres_givens(int B, int nblk,
                                                              double* __restrict__ st, int nq) {
   __shared__ double red[kThreads];
   const int j = blockIdx.x;
-  const double s = block_sum_col(partial, nblk, B, j, red);
-  double q2 = 0.0, q3 = 0.0;
-  if (nq >= 3) {
+  double s, q2 = 0.0, q3 = 0.0;
+  if (nq >= 3) {  // fused steps: [k][column][block] partials
+    s = block_sum_contig(partial + (int64_t)j * nblk, nblk, red);
     __syncthreads();
-    q2 = block_sum_col(partial + (int64_t)nblk * B, nblk, B, j, red);
+    q2 = block_sum_contig(partial + ((int64_t)B + j) * nblk, nblk, red);
     __syncthreads();
-    q3 = block_sum_col(partial + (int64_t)2 * nblk * B, nblk, B, j, red);
+    q3 = block_sum_contig(partial + ((int64_t)2 * B + j) * nblk, nblk, red);
+  } else {
+    s = block_sum_col(partial, nblk, B, j, red);
   }
   if (threadIdx.x != 0) return;
   double* S = st;
@@ -2140,6 +2162,23 @@ __global__ __launch_bounds__(kThreads) void k_pairdot(int64_t N, int B, const do
     for (int m = 0; m < per; ++m) t += red[m * np + tid];
     partial[(int64_t)blockIdx.x * np + tid] = t;
   }
+}
+
+// Attainable FMA rate: `CH` independent multiply-add chains per thread,
+// `iters` rounds, one store per thread (out holds one value per thread).
+template <typename T, int CH>
+__global__ __launch_bounds__(kThreads) void k_fma_probe(T* __restrict__ out, T a, T b, int iters) {
+  T acc[CH];
+#pragma unroll
+  for (int c = 0; c < CH; ++c) acc[c] = (T)(threadIdx.x + c);
+  for (int i = 0; i < iters; ++i) {
+#pragma unroll
+    for (int c = 0; c < CH; ++c) acc[c] = acc[c] * a + b;
+  }
+  T s = 0;
+#pragma unroll
+  for (int c = 0; c < CH; ++c) s += acc[c];
+  out[(size_t)blockIdx.x * kThreads + threadIdx.x] = s;
 }
 
 inline int grid_for(int64_t n) {
@@ -2415,6 +2454,7 @@ int launch_minres_gather(const Geo& g, const TY* Yc, const double* S, const doub
   uint32_t R;
   const int vw = gather_vw(B, (int)sizeof(double));
   gather_grid(g, B, vw, gather_rows, gx, gy, R);
+  gx = std::min<unsigned>(gx, (unsigned)kMinresGatherBlocks);
 #define IDFT_MG(PP, VW)                                                                   \
   do {                                                                                     \
     if (w)                                                                                 \
@@ -2731,6 +2771,48 @@ int launch_pairdot_partial(int64_t N, int B, const double* X, const double* Y, c
   k_pairdot<<<nb, kThreads, 0, s>>>(N, B, X, Y, w, pairs, np, partial);
   IDFT_HIP(hipGetLastError());
   return nb;
+}
+
+
+// ------------------------------------------------------ FMA rate probe
+template <typename T, int CH>
+static double fma_probe_t(hipStream_t s) {
+  int dev = 0, ncu = 0;
+  IDFT_HIP(hipGetDevice(&dev));
+  IDFT_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  const int blocks = std::max(1, ncu) * 32, iters = 2048;
+  T* out = nullptr;
+  IDFT_HIP(hipMalloc(&out, sizeof(T) * (size_t)blocks * kThreads));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  float ms = 0.0f;
+  hipError_t err = hipEventCreate(&e0);
+  if (err == hipSuccess) err = hipEventCreate(&e1);
+  if (err == hipSuccess) {
+    k_fma_probe<T, CH><<<blocks, kThreads, 0, s>>>(out, (T)0.999, (T)0.001, iters);
+    err = hipGetLastError();
+  }
+  if (err == hipSuccess) err = hipEventRecord(e0, s);
+  if (err == hipSuccess) {
+    k_fma_probe<T, CH><<<blocks, kThreads, 0, s>>>(out, (T)0.999, (T)0.001, iters);
+    err = hipGetLastError();
+  }
+  if (err == hipSuccess) err = hipEventRecord(e1, s);
+  if (err == hipSuccess) err = hipEventSynchronize(e1);
+  if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(out);
+  IDFT_HIP(err);
+  const double flops = 2.0 * CH * (double)iters * blocks * kThreads;
+  return flops / ((double)ms * 1e-3) / 1e12;
+}
+double launch_fma_probe(bool fp64, int chains, hipStream_t s) {
+  switch (chains) {
+    case 4: return fp64 ? fma_probe_t<double, 4>(s) : fma_probe_t<float, 4>(s);
+    case 8: return fp64 ? fma_probe_t<double, 8>(s) : fma_probe_t<float, 8>(s);
+    case 16: return fp64 ? fma_probe_t<double, 16>(s) : fma_probe_t<float, 16>(s);
+    default: throw Error(IDFT_EINVAL, "chains must be 4, 8 or 16");
+  }
 }
 
 }  // namespace idft
