@@ -69,11 +69,14 @@ def sf_flops_per_cell_column(p):
 # HBM bytes per launch of the cell-product kernels from two rocprofv3 --pmc runs
 # (FETCH_SIZE, WRITE_SIZE) of `bench.py --kernels-only --shape ...`, FETCH_SIZE
 # scaled by the copy calibration (x1.926), WRITE_SIZE as read; the summaries
-# with the commands are profiles/cell_sf_inverse_24c_p5_pmc.md (shape 1 4 4 2)
-# and profiles/cell_gemm_inverse_24c_p5_pmc.md (2 16 4 0 and 1 16 0 1).
+# with the commands are profiles/cell_sf_inverse_24c_p5_pmc.md (shape 1 4 4 2),
+# profiles/cell_sf_batched_40c_p4_pmc.md (1 8 8 2) and
+# profiles/cell_gemm_inverse_24c_p5_pmc.md (2 16 4 0 and 1 16 0 1).
 MEASURED_TRAFFIC = {
     ("inverse_24c_p5", "k_cell_sf<float,float,5,4,1,false>"): 276.3e6,
-    ("inverse_24c_p5", "k_cell_sf<double,double,5,4,1,false>"): 552.5e6,
+    ("inverse_24c_p5", "k_cell_sf<double,double,5,4,1,false>"): 552.4e6,
+    ("batched_40c_p4", "k_cell_sf<float,float,4,8,1,false>"): 4245.6e6,
+    ("batched_40c_p4", "k_cell_sf<double,double,4,8,1,false>"): 9264.6e6,
     ("inverse_24c_p5", "k_cell_gemm<float,float,5,1>"): 2613e6,
     ("inverse_24c_p5", "k_cell_symm<float,float,5,1>"): 1691e6,
 }
@@ -134,7 +137,8 @@ def run_bfgs(lib, prob, precision, warmup, steps, tol_kw):
 
 
 def _oracle_samples(olib, prob):
-    """Oracle H_e build and one n_eig-column Loewdin apply on prob's mesh (s)."""
+    """Oracle H_e build and one n_eig-column Loewdin apply (median of three)
+    on prob's mesh (s)."""
     ctx = prob.context(olib)
     s = ctx.sizes
     t0 = time.perf_counter()
@@ -142,9 +146,13 @@ def _oracle_samples(olib, prob):
     t_build = time.perf_counter() - t0
     B = prob.desc.n_eig
     x = np.random.default_rng(0).standard_normal((s.n_nodes, B))
-    t0 = time.perf_counter()
-    ctx.apply_hamiltonian(x)
-    t_apply = time.perf_counter() - t0
+    # median of three applications: one is too noisy on shared host cores
+    ts = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        ctx.apply_hamiltonian(x)
+        ts.append(time.perf_counter() - t0)
+    t_apply = float(np.median(ts))
     ctx.close()
     return t_build, t_apply
 
@@ -181,7 +189,7 @@ def cpu_baseline(lib, prob, per_eval, evals_per_iter, tol_kw):
     return {"value": round(t_eval * evals_per_iter, 3), "unit": "s/iter", "cores": cores,
             "kind": "port",
             "sample": (f"FP64 oracle on this mesh: H_e build {t_build:.2f} s + one {B}-column "
-                       f"Loewdin apply {t_apply:.2f} s, scaled to {cols:.0f} column-applies and "
+                       f"Loewdin apply {t_apply:.2f} s (median of 3), scaled to {cols:.0f} column-applies and "
                        f"2 cell-matrix builds per evaluation, {evals_per_iter} evaluations per "
                        f"iteration"),
             "measured_check": {"workload": "adjoint_16c_p4", "s_per_eval_measured": round(t_meas, 3),
@@ -240,9 +248,11 @@ def c3_leg(lib, cheb_tol):
         nbytes = 8 * (cells * (p + 3) ** 3 + N * (B + 1) + cells * (p + 1) ** 3 * B)
         ms = out["fp64"]["fp64_cell_product_ms"]
         ach = flops / (ms * 1e-3) / 1e12
-        roof = {"bound": "valu", "kernel": f"k_cell_sf<double,double,{p},{sh['vectors_per_tile']},1,false>",
+        kname = f"k_cell_sf<double,double,{p},{sh['vectors_per_tile']},1,false>"
+        roof = {"bound": "valu", "kernel": kname,
                 "achieved": round(ach, 2), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(ach / FP64_PEAK_TFLOPS, 4), "ms_per_launch": ms,
+                "frac": round(ach / FP64_PEAK_TFLOPS, 4),
+                "traffic": MEASURED_TRAFFIC.get(("batched_40c_p4", kname)), "ms_per_launch": ms,
                 "flops_per_launch": flops, "bytes_per_launch": nbytes,
                 "achieved_GBs": round(nbytes / (ms * 1e-3) / 1e9, 1)}
     return {"workload": "batched_40c_p4", "nodes": int(prob.desc.n_cells * prob.desc.degree + 1) ** 3,
@@ -496,11 +506,9 @@ def main():
         force_err = float(np.linalg.norm(fm - ff) / np.linalg.norm(ff))
         log(f"force rel err mixed vs FP64 (tight solves): {force_err:.3e}; loss {lm:.8e} vs {lf:.8e}")
 
-    # roofline of the dominant kernel: the FP64 cell product (phase 1 of the
-    # operator) of the mixed run, which applies the operator in the MINRES steps
-    # before the switch to FP32 and takes the largest share of the iteration's
-    # GPU time; the FP32 cell product of the Chebyshev filter is reported beside
-    # it.  For the kernel the autotuner chose:
+    # roofline of the dominant kernel: the cell product (phase 1 of the
+    # operator) of the mixed run at each precision.  For the kernel the
+    # autotuner chose:
     #  - stored-H kernels (k_cell_gemm, k_cell_symm) are HBM-bound; their
     #    algorithmic bytes per launch are the symmetric H_e (ne(ne+1)/2 entries
     #    per cell) read once, the block X and node scale S read once and the
@@ -548,17 +556,23 @@ def main():
         r["operator_ms"] = round(ms_op, 4)
         return r
 
-    ms_op64, ms_cell64 = cm.profile_apply(B, abi.FP64, reps=20)
-    roof = roofline(mixed["tuned"]["fp64"], False, ms_cell64, ms_op64)
+    # The dominant kernel of the mixed iteration is the FP32 cell product of
+    # the Chebyshev filter (the largest share of kernel time in the iteration
+    # trace); the FP64 cell product of the MINRES steps and Rayleigh-Ritz is
+    # reported beside it.
     ms_op32, ms_cell32 = cm.profile_apply(B, abi.MIXED, reps=20)
-    extra = {"fp32_cell_product": roofline(mixed["tuned"]["fp32"], True, ms_cell32, ms_op32)}
-    # attainable FMA rates of this GPU (16 independent chains per thread):
-    # the rate a latency-limited kernel can approach, beside the published
-    # peaks that `frac` uses
+    roof = roofline(mixed["tuned"]["fp32"], True, ms_cell32, ms_op32)
+    ms_op64, ms_cell64 = cm.profile_apply(B, abi.FP64, reps=20)
+    extra = {"fp64_cell_product": roofline(mixed["tuned"]["fp64"], False, ms_cell64, ms_op64)}
+    # attainable FMA rates of this GPU (16 independent chains per thread,
+    # unpacked FP32): the rate a latency- or issue-limited kernel can
+    # approach, beside the published peaks that `frac` uses
     fma = fma_probe(lib)
     extra["attainable_fma_tflops"] = fma
-    if roof.get("unit") == "TFLOP/s" and fma.get("fp64"):
-        roof["attainable_fma"] = fma["fp64"]
+    for r, key in ((roof, "fp32"), (extra["fp64_cell_product"], "fp64")):
+        if r.get("unit") == "TFLOP/s" and fma.get(key):
+            r["attainable_fma"] = fma[key]
+            r["frac_of_attainable"] = round(r["achieved"] / fma[key], 4)
     if cf is not None:
         m64, g64 = cf.profile_apply(B, abi.FP64, reps=20)
         extra["fp64_run_cell_product"] = {"ms_per_launch": round(g64, 4),

@@ -43,9 +43,17 @@ __device__ inline double block_sum_col(const double* __restrict__ partial, int n
 // Sum of n contiguous values by one work-group (all threads call it).
 __device__ inline double block_sum_contig(const double* __restrict__ v, int n, double* red) {
   const int tid = threadIdx.x;
-  double s = 0.0;
-  for (int b = tid; b < n; b += kThreads) s += v[b];
-  red[tid] = s;
+  // four independent partial sums, so that four loads are in flight
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  int b = tid;
+  for (; b + 3 * kThreads < n; b += 4 * kThreads) {
+    s0 += v[b];
+    s1 += v[b + kThreads];
+    s2 += v[b + 2 * kThreads];
+    s3 += v[b + 3 * kThreads];
+  }
+  for (; b < n; b += kThreads) s0 += v[b];
+  red[tid] = (s0 + s1) + (s2 + s3);
   __syncthreads();
   for (int st = kThreads / 2; st > 0; st >>= 1) {
     if (tid < st) red[tid] += red[tid + st];
