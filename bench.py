@@ -105,6 +105,26 @@ def fma_probe(lib, chains=16):
     return out
 
 
+def mixed_vs_fp64_errors(lib, prob, x, tol_kw):
+    """getForceVector at potential x in both precisions with the given
+    tolerances: relative force and density (L2 over the quadrature values)
+    errors, the largest eigenvalue difference over the occupied states (Ha)
+    and the relative loss difference of mixed against FP64."""
+    from invdft_amd import abi
+    out = {}
+    for prec in (abi.MIXED, abi.FP64):
+        c = prob.context(lib, precision=prec, **tol_kw)
+        f, L = c.get_force(x)
+        out[prec] = (f, L, c.eigenvalues(), c.density())
+        c.close()
+    (fm, lm, em, rm), (ff, lf, ef, rf) = out[abi.MIXED], out[abi.FP64]
+    nocc = int(round(prob.desc.n_electrons / 2))
+    return {"force": float(np.linalg.norm(fm - ff) / np.linalg.norm(ff)),
+            "eigenvalues_occupied_Ha": float(np.max(np.abs(em[:nocc] - ef[:nocc]))),
+            "density_l2": float(np.linalg.norm(rm - rf) / np.linalg.norm(rf)),
+            "loss": float(abs(lm - lf) / abs(lf))}
+
+
 def run_bfgs(lib, prob, precision, warmup, steps, tol_kw):
     from invdft_amd import abi
     ctx = prob.context(lib, precision=precision, **tol_kw)
@@ -208,7 +228,7 @@ def c3_leg(lib, cheb_tol):
     prob = synthetic.make_problem(lib, seed=2, target_tol=1e-10, **cfg)
     log(f"  C3 set-up: {time.perf_counter() - t0:.1f} s")
     kw = dict(cheb_tol=cheb_tol, initial_tol_cheb=cheb_tol)
-    out, forces = {}, {}
+    out, forces, states = {}, {}, {}
     B = prob.desc.n_eig
     for name, prec in (("mixed", abi.MIXED), ("fp64", abi.FP64)):
         ctx = prob.context(lib, precision=prec, **kw)
@@ -220,6 +240,7 @@ def c3_leg(lib, cheb_tol):
         t0 = time.perf_counter()
         f0, l0 = ctx.get_force(x0.copy())
         st0 = ctx.stats()
+        states[name] = (ctx.eigenvalues(), ctx.density())
         f1, l1 = ctx.get_force(x0 - f0)
         dt = time.perf_counter() - t0
         st1 = ctx.stats()
@@ -237,6 +258,10 @@ def c3_leg(lib, cheb_tol):
         ctx.close()
     fm, ff = forces["mixed"], forces["fp64"]
     err = [float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in zip(fm, ff)]
+    nocc = int(round(prob.desc.n_electrons / 2))
+    (em, rm), (ef, rf) = states["mixed"], states["fp64"]
+    state_err = {"eigenvalues_occupied_Ha": float(np.max(np.abs(em[:nocc] - ef[:nocc]))),
+                 "density_l2": float(np.linalg.norm(rm - rf) / np.linalg.norm(rf))}
     # roofline of this workload's FP64 cell product (the FP64 run's tuned
     # shape), measured live by idft_profile_apply as for the headline kernel
     roof = None
@@ -258,7 +283,8 @@ def c3_leg(lib, cheb_tol):
     return {"workload": "batched_40c_p4", "nodes": int(prob.desc.n_cells * prob.desc.degree + 1) ** 3,
             "orbitals": B, "evals_per_iter": 2, "mixed": out["mixed"], "fp64": out["fp64"],
             "speedup_mixed_vs_fp64": round(out["fp64"]["s_per_iter"] / out["mixed"]["s_per_iter"], 3),
-            "force_rel_err_mixed_vs_fp64": err, "roofline": roof}
+            "force_rel_err_mixed_vs_fp64": err,
+            "state_err_mixed_vs_fp64_first_eval": state_err, "roofline": roof}
 
 
 def c4_large(lib, evals):
@@ -493,18 +519,17 @@ def main():
     # force error of the mixed path against FP64 at the same potential, both
     # solved tightly (eigen residual 1e-10, adjoint 1e-13) so that the
     # difference measures precision, not solver truncation
+    # The workload also states the eigenvalue and density errors; the same
+    # comparison at the timed run's own tolerances is reported beside it.
     force_err = None
+    errors = None
     if cf is not None:
         x = cm.get_solution()
         tight = dict(cheb_tol=1e-10, initial_tol_cheb=1e-10)
-        fs = {}
-        for prec in (abi.MIXED, abi.FP64):
-            c = prob.context(lib, precision=prec, **tight)
-            fs[prec] = c.get_force(x)
-            c.close()
-        (fm, lm), (ff, lf) = fs[abi.MIXED], fs[abi.FP64]
-        force_err = float(np.linalg.norm(fm - ff) / np.linalg.norm(ff))
-        log(f"force rel err mixed vs FP64 (tight solves): {force_err:.3e}; loss {lm:.8e} vs {lf:.8e}")
+        errors = {"tight": mixed_vs_fp64_errors(lib, prob, x, tight),
+                  "timed_tolerances": mixed_vs_fp64_errors(lib, prob, x, tol_kw)}
+        force_err = errors["tight"]["force"]
+        log(f"mixed vs FP64 at the final potential: {errors}")
 
     # roofline of the dominant kernel: the cell product (phase 1 of the
     # operator) of the mixed run at each precision.  For the kernel the
@@ -607,6 +632,7 @@ def main():
         "fp64_s_per_iter": round(fp64["s_per_iter"], 4) if fp64 else None,
         "speedup_mixed_vs_fp64": round(fp64["s_per_iter"] / mixed["s_per_iter"], 3) if fp64 else None,
         "force_rel_err_mixed_vs_fp64": force_err,
+        "errors_mixed_vs_fp64": errors,
         "roofline": roof,
         "cpu_baseline": cpu,
         "detail": {"mixed_per_eval": mixed["per_eval"], "fp64_per_eval": fp64["per_eval"] if fp64 else None,

@@ -16,6 +16,27 @@ def test_backend(glib):
     assert glib.idft_backend() == b"hip-gfx950"
 
 
+def test_fma_probe(glib):
+    """idft_fma_probe: positive rates below the published peaks (FP64 78.6,
+    unpacked FP32 78.6 TFLOP/s), more independent chains no slower than
+    fewer by more than noise, and IDFT_EINVAL for unsupported chain counts."""
+    import ctypes as C
+    _say("fma_probe")
+    fn = glib.idft_fma_probe
+    fn.restype = C.c_int
+    fn.argtypes = [C.c_int32, C.c_int32, C.POINTER(C.c_double)]
+    v = C.c_double()
+    rates = {}
+    for prec in (abi.FP64, abi.MIXED):
+        for ch in (4, 16):
+            assert fn(prec, ch, C.byref(v)) == 0
+            rates[(prec, ch)] = v.value
+            assert 1.0 < v.value < 80.0
+        assert rates[(prec, 16)] > 0.8 * rates[(prec, 4)]
+    print(f"  {rates}", flush=True)
+    assert fn(abi.FP64, 5, C.byref(v)) == abi.IDFT_EINVAL
+
+
 @pytest.mark.parametrize("p", [2, 3, 4, 5])
 @pytest.mark.parametrize("ncols", [1, 12, 20, 80])
 def test_cell_apply_fp64(glib, olib, p, ncols):
