@@ -528,8 +528,9 @@ __global__ __launch_bounds__(kSymmThreads) void k_cell_symm(Geo g, const T* __re
 // even-odd form below: 11.9 k FMA and 6.6 k adds per column at p = 5 (23 k FMA
 // without the even-odd split, 47 k for the dense product), and per cell only
 // the (p+3)^3 weights w_e are read from HBM instead of n_e(n_e+1)/2 entries of
-// H_e.  The 1D tables are compile-time constants (SfC<P>), so after unrolling
-// every coefficient is an immediate operand; h/4 scales the kinetic sum once.
+// H_e.  The 1D tables are compile-time constants (SfC<P>): after unrolling
+// every FP32 coefficient is an immediate operand, and the FP64 ones come from
+// a constant-memory table (eo_coef below); h/4 scales the kinetic sum once.
 //
 // One work-group handles one cell and CG columns; an item is one line of the
 // cell's tensor along one axis for VC consecutive columns (VC-wide LDS
