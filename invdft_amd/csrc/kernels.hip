@@ -788,6 +788,18 @@ __global__ __launch_bounds__(kThreads) void k_cell_sf(Geo g, T hq, const T* __re
   const int tid = threadIdx.x;
   const uint32_t cx = e % nc, cy = (e / nc) % nc, cz = e / (nc * nc);
   const T* we = W + e * (uint32_t)(Q2 * Q);
+  // the quadrature weights of this thread's stage-6 items, loaded now so that
+  // their latency hides behind stages 1-5 (item it6 = tid + k6 kThreads takes
+  // w_e[qz][ln6] for its line ln6 = it6 / NV)
+  constexpr int K6 = (Q2 * NV + kThreads - 1) / kThreads;
+  T wpre[K6][Q];
+#pragma unroll
+  for (int k6 = 0; k6 < K6; ++k6) {
+    const int it6 = tid + k6 * kThreads;
+    const int ln6 = it6 < Q2 * NV ? it6 / NV : 0;
+#pragma unroll
+    for (int qz = 0; qz < Q; ++qz) wpre[k6][qz] = we[qz * Q2 + ln6];
+  }
   // the input lines stay in registers from stage 1 to stage 4 (even-odd split)
   pk keep_p[KI][(N1 + 1) / 2], keep_m[KI][N1 / 2];
   // stage 1: x lines ln = c*N1 + b of the gathered input -> ka = M x, kb = D x
@@ -920,7 +932,10 @@ __global__ __launch_bounds__(kThreads) void k_cell_sf(Geo g, T hq, const T* __re
   __syncthreads();
   // stage 6: z lines (qy, qx) through the quadrature weights, in place:
   // s2 = Lq^T (w .* (Lq t2)) (over t2)
-  for (int it = tid; it < Q2 * NV; it += kThreads) {
+#pragma unroll
+  for (int k6 = 0; k6 < K6; ++k6) {
+    const int it = tid + k6 * kThreads;
+    if (it >= Q2 * NV) break;
     const int jv = it % NV, ln = it / NV;  // ln = qy * Q + qx
     const int qx = ln % Q, qy = ln / Q;
     pk v[N1], vp[(N1 + 1) / 2], vm[N1 / 2];
@@ -930,7 +945,7 @@ __global__ __launch_bounds__(kThreads) void k_cell_sf(Geo g, T hq, const T* __re
     pk u[Q], up[(Q + 1) / 2], um[Q / 2];
     eo_out<Q, T, VC>([&](auto i, pk& s, pk& d) { eo_row<T, TL, IDFT_IC(i)>(vp, vm, s, d); },
                      [&](auto qz, const pk& r) {
-                       const T wq = we[IDFT_IC(qz) * Q2 + ln];
+                       const T wq = wpre[k6][IDFT_IC(qz)];
 #pragma unroll
                        for (int uu = 0; uu < VC; ++uu) u[IDFT_IC(qz)].v[uu] = r.v[uu] * wq;
                      });
