@@ -788,6 +788,15 @@ __global__ __launch_bounds__(kThreads) void k_cell_sf(Geo g, T hq, const T* __re
   const int tid = threadIdx.x;
   const uint32_t cx = e % nc, cy = (e / nc) % nc, cz = e / (nc * nc);
   const T* we = W + e * (uint32_t)(Q2 * Q);
+  // the column scales of this thread's columns (every item of a thread has
+  // column pack jv = tid % NV, since kThreads is a multiple of NV), loaded
+  // first: a later load would make their first use wait for the weights too
+  double xcl[VC];
+#pragma unroll
+  for (int u = 0; u < VC; ++u) {
+    const int j = min((tid % NV) * VC + u, bw - 1);
+    xcl[u] = xcol ? xcol[c0 + j] : 1.0;
+  }
   // the quadrature weights of this thread's stage-6 items, loaded now so that
   // their latency hides behind stages 1-5 (item it6 = tid + k6 kThreads takes
   // w_e[qz][ln6] for its line ln6 = it6 / NV)
@@ -821,7 +830,7 @@ __global__ __launch_bounds__(kThreads) void k_cell_sf(Geo g, T hq, const T* __re
       const int j = jv * VC + u;
       if (j < bw) {
         const TV* Xp = X + (size_t)n0 * ub + (uint32_t)(c0 + j);
-        const double xc = xcol ? xcol[c0 + j] : 1.0;
+        const double xc = xcl[u];
         T sv[N1];
         TV xv[N1];
         double nv[N1];
