@@ -4,6 +4,7 @@
   * compaction: unoccupied states (f = 0, so b~ = 0) never enter the MINRES
     block and converged columns leave it, so the streamed column-steps are
     well below steps x block width, while the force still equals the oracle's;
+    the repacked width is kept even (one finished column rides along);
   * checkpoint (desc.psi_checkpoint = 1): the eigenvector block is kept as an
     FP32 copy between the passes and recomputed by one ChFSI pass for the
     backward pass; the force equals the resident path's to the eigensolver
@@ -52,6 +53,28 @@ def test_compaction_streams_open_columns_only(glib, olib, prec):
     assert st["minres_compactions"] >= 1
     assert st["minres_column_steps"] <= 2 * st["minres_iters_max"] + 8 * B
     assert rel(fg, fo) < (1e-7 if prec == abi.FP64 else 1e-6)
+    g.close()
+
+
+def test_compaction_keeps_even_width(glib, olib):
+    """One occupied state of eight: after the first check one column is open,
+    and the repacked block keeps one finished column as an inert second (even
+    width); the force still equals the oracle's, and the column-steps stay
+    those of the open column plus that one."""
+    _say("compaction even width")
+    prob = small_problem(olib, n_electrons=2.0)
+    kw = dict(cheb_tol=1e-11, initial_tol_cheb=1e-11)
+    g = prob.context(glib, **kw)
+    fg, lg = g.get_force(prob.vxc0)
+    st = g.stats()
+    fo, lo = prob.context(olib, **kw).get_force(prob.vxc0)
+    B = prob.desc.n_eig
+    print(f"  column-steps {st['minres_column_steps']} vs {st['minres_iters_max']} x {B}, "
+          f"compactions {st['minres_compactions']}", flush=True)
+    assert st["minres_compactions"] >= 1
+    assert st["minres_column_steps"] <= 2 * st["minres_iters_max"] + 8 * B
+    assert abs(lg - lo) < 1e-9 * abs(lo)
+    assert rel(fg, fo) < 1e-7
     g.close()
 
 
