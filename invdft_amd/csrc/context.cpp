@@ -884,7 +884,8 @@ struct idft_ctx {
     bool f32op = false;
     // one MINRES step with a stored-H cell product: 5 launches, all arguments
     // device-resident (scalars in mst, done flags per column), so a run of
-    // steps can be replayed as a graph.  The two buffer swaps return to the start after two steps.
+    // steps can be replayed as a graph.  The two buffer swaps return to the
+    // start after two steps.
     // N*bc streams per step: 1 (cell product) + 3 (gather) + 3 (Lanczos) + 6
     // (update).  r2 = r_k (newest Lanczos residual), r1 = r_{k-1}; the step's
     // Lanczos vector v = cv D r2 is formed on the fly (not stored).
