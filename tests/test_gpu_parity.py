@@ -237,7 +237,7 @@ def test_minres_fused_step_matches_separate(glib, olib, prec):
     assert np.all(np.abs(out["fused"][1] - its_o) <= 3)
 
 
-@pytest.mark.parametrize("p", [4, 5])
+@pytest.mark.parametrize("p", [2, 3, 4, 5])
 @pytest.mark.parametrize("mode", [abi.GRADIENT_REFERENCE, abi.GRADIENT_EXACT])
 def test_get_force(glib, olib, mode, p):
     _say("get_force", mode, p)
@@ -251,7 +251,7 @@ def test_get_force(glib, olib, mode, p):
     assert rel(fg, fo) < 1e-7
 
 
-@pytest.mark.parametrize("p", [4, 5])
+@pytest.mark.parametrize("p", [2, 3, 4, 5])
 def test_mixed_precision_force(glib, olib, p):
     _say("mixed_force", p)
     prob = small_problem(olib, degree=p)
