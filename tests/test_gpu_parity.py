@@ -102,10 +102,11 @@ def _adjoint_setup(ctx, prob, psi, occ, eig, u, pairs, fermi):
     return mpsi, neg
 
 
-@pytest.mark.parametrize("p", [4, 5])
+@pytest.mark.parametrize("p", [2, 3, 4, 5])
 def test_adjoint_contract(glib, olib, p):
-    """Every contract-2 entry point at degree 4 and 5 (the node-scaled
-    sum-factorised cell products of the MINRES steps at both)."""
+    """Every contract-2 entry point at degrees 2 to 5 (the node-scaled
+    sum-factorised cell products and fused gathers of the MINRES steps at
+    each)."""
     _say("adjoint_contract", p)
     prob = small_problem(olib, degree=p)
     o = prob.context(olib)
