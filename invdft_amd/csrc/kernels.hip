@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <mutex>
 #include <type_traits>
 
@@ -24,43 +25,56 @@ namespace {
 
 constexpr int kThreads = 256;
 
-// Sum of partial[b * B + j] over b < nblk by one 256-thread block (fixed
-// order: strided per-thread sums, then an LDS tree), result on thread 0.
+// Total of one value per thread over a work-group of NT threads (fixed
+// order: each wave by shuffles, then the NT / 64 wave sums in order), returned
+// to every thread.  red holds NT / 64 doubles; a caller that reuses red for a
+// second total synchronises in between.
+template <int NT>
+__device__ inline double block_total(double t, double* red) {
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+  if ((tid & 63) == 0) red[tid >> 6] = t;
+  __syncthreads();
+  double s = 0.0;
+#pragma unroll
+  for (int w = 0; w < NT / 64; ++w) s += red[w];
+  return s;
+}
+
+// Sum of partial[b * B + j] over b < nblk by one work-group of NT threads
+// (fixed order: strided per-thread sums, then block_total).
+template <int NT = kThreads>
 __device__ inline double block_sum_col(const double* __restrict__ partial, int nblk, int B, int j,
                                        double* red) {
   const int tid = threadIdx.x;
   double s = 0.0;
-  for (int b = tid; b < nblk; b += kThreads) s += partial[(int64_t)b * B + j];
-  red[tid] = s;
-  __syncthreads();
-  for (int st = kThreads / 2; st > 0; st >>= 1) {
-    if (tid < st) red[tid] += red[tid + st];
-    __syncthreads();
-  }
-  return red[0];
+  for (int b = tid; b < nblk; b += NT) s += partial[(int64_t)b * B + j];
+  return block_total<NT>(s, red);
 }
 
-// Sum of n contiguous values by one work-group (all threads call it).
+// Sum of n contiguous values by one work-group of NT threads (all threads
+// call it).
+template <int NT = kThreads>
 __device__ inline double block_sum_contig(const double* __restrict__ v, int n, double* red) {
   const int tid = threadIdx.x;
   // four independent partial sums, so that four loads are in flight
   double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
   int b = tid;
-  for (; b + 3 * kThreads < n; b += 4 * kThreads) {
+  for (; b + 3 * NT < n; b += 4 * NT) {
     s0 += v[b];
-    s1 += v[b + kThreads];
-    s2 += v[b + 2 * kThreads];
-    s3 += v[b + 3 * kThreads];
+    s1 += v[b + NT];
+    s2 += v[b + 2 * NT];
+    s3 += v[b + 3 * NT];
   }
-  for (; b < n; b += kThreads) s0 += v[b];
-  red[tid] = (s0 + s1) + (s2 + s3);
-  __syncthreads();
-  for (int st = kThreads / 2; st > 0; st >>= 1) {
-    if (tid < st) red[tid] += red[tid + st];
-    __syncthreads();
-  }
-  return red[0];
+  for (; b < n; b += NT) s0 += v[b];
+  return block_total<NT>((s0 + s1) + (s2 + s3), red);
 }
+
+// Work-group size of the per-column reductions of long partial rows (cell
+// dots, gather partials, Gram partials): 16 waves, about 12 values per thread
+// for the 12-16 k values of a column.
+constexpr int kRedThreads = 1024;
 
 inline void check_u32(int64_t n) {
   if (n >= (int64_t)0xffffffffLL) throw Error(IDFT_EINVAL, "multivector too large for 32-bit indexing");
@@ -537,7 +551,15 @@ __global__ __launch_bounds__(kSymmThreads) void k_cell_symm(Geo g, const T* __re
 // accesses).  Intermediates live in LDS as [entry][CG / VC] packs, with padded
 // line strides (N1 and Q rounded up to odd: 5, 7 at p = 4; 7, 9 at p = 5) so
 // that the 64 lanes of a wave spread over the banks (an even stride such as 6
-// or 8 at p = 4 folds the lines of a wave onto a few banks).  The kinetic
+// or 8 at p = 4 folds the lines of a wave onto a few banks).  The exception is
+// region D in FP64 at p = 5 (Q = 8): unpadded lines, entry x of line (c, y)
+// at slot x ^ y, which spreads the lanes over the banks as the stride 9 does,
+// and puts the lines that one stage-6 item reads 512 B apart, so that the
+// compiler pairs those accesses (ds_read2st64 / ds_write2st64).  It also
+// takes 20.4 KB instead of 21.9 KB per work-group, which does not raise the
+// occupancy: the kernel's 70 registers allow 7 waves per SIMD either way (with
+// the stage-6 weights loaded after stage 3 it fits 64 registers and 8
+// work-groups per CU, but measured slower: the weights' latency shows).  The kinetic
 // chain runs first and the x-line input stays in registers for the potential
 // chain, and the y and z stages overwrite their input in place (each item
 // reads its whole line before writing the same positions; in stage 5 the
@@ -552,8 +574,8 @@ __global__ __launch_bounds__(kSymmThreads) void k_cell_symm(Geo g, const T* __re
 //   6  z lines (qy,qx): s2 = Lq^T (w .* (Lq t2)) -> D
 //   7  y lines (c,qx): s1[c][b][qx] = Lq^T s2 -> D
 //   8  x lines (c,b):  Yc[e][(c,b,a)][j] = (Lq^T s1)[a] + (h/4) yk[a]
-// At p = 5 that is 252 + 432 = 684 entries per column: 21.9 KB per work-group
-// of 4 FP64 columns, 7 work-groups (28 waves) per CU.
+// At p = 5 that is 252 + 432 = 684 entries per column, 252 + 384 = 636 with
+// the unpadded FP64 region D.
 __host__ __device__ constexpr int cmax(int a, int b) { return a > b ? a : b; }
 
 template <typename T, int VC>
@@ -654,26 +676,24 @@ constexpr EoTable<P> make_eo_table() {
   return t;
 }
 // Written from the host once per device (upload_sf_constants): with an
-// initialiser the compiler would fold the values back into immediates.
+// initialiser the compiler would fold the values back into immediates.  The
+// kernels take the table's device address as an argument (sf_eo_table): an
+// access through the symbol itself goes through the global offset table, one
+// more dependent scalar load wherever the compiler reloads the coefficients.
 __constant__ EoTable<2> kEo2;
 __constant__ EoTable<3> kEo3;
 __constant__ EoTable<4> kEo4;
 __constant__ EoTable<5> kEo5;
-template <int P> __device__ inline const EoTable<P>& eo_table();
-template <> __device__ inline const EoTable<2>& eo_table<2>() { return kEo2; }
-template <> __device__ inline const EoTable<3>& eo_table<3>() { return kEo3; }
-template <> __device__ inline const EoTable<4>& eo_table<4>() { return kEo4; }
-template <> __device__ inline const EoTable<5>& eo_table<5>() { return kEo5; }
 // Coefficient W (0 even, 1 odd, 2 middle) of row I, column J of operator A:
-// an immediate in FP32, a constant-memory load in FP64.
+// an immediate in FP32, a scalar load from the table eo (EoTable<P>::v) in FP64.
 template <typename T, class A, int I, int J, int W>
-__device__ inline T eo_coef() {
+__device__ inline T eo_coef(const double* __restrict__ eo) {
   if constexpr (std::is_same<T, double>::value) {
     using E = EoTable<A::kP>;
     constexpr int c = E::C(A::kK);
     constexpr int idx = E::base(A::kK) + I * E::stride(A::kK) +
                         (W == 0 ? J : W == 1 ? c / 2 + J : 2 * (c / 2));
-    return eo_table<A::kP>().v[idx];
+    return eo[idx];
   } else if constexpr (W == 0) {
     return eo_even<T, A, I, J>();
   } else if constexpr (W == 1) {
@@ -698,13 +718,13 @@ __device__ inline void eo_split(const Pack<T, VC> (&x)[C], Pack<T, VC> (&xp)[(C 
 template <typename T, class A, int I, int VC>
 __device__ inline void eo_row(const Pack<T, VC> (&xp)[(A::C + 1) / 2],
                               const Pack<T, VC> (&xm)[A::C / 2], Pack<T, VC>& e,
-                              Pack<T, VC>& o) {
+                              Pack<T, VC>& o, const double* __restrict__ eo) {
   static_for<0, A::C / 2>([&](auto j) {
     constexpr int J = decltype(j)::value;
-    pack_fma(e, eo_coef<T, A, I, J, 0>(), xp[J]);
-    if constexpr (2 * I + 1 != A::R) pack_fma(o, eo_coef<T, A, I, J, 1>(), xm[J]);
+    pack_fma(e, eo_coef<T, A, I, J, 0>(eo), xp[J]);
+    if constexpr (2 * I + 1 != A::R) pack_fma(o, eo_coef<T, A, I, J, 1>(eo), xm[J]);
   });
-  if constexpr (A::C % 2 == 1) pack_fma(e, eo_coef<T, A, I, 0, 2>(), xp[A::C / 2]);
+  if constexpr (A::C % 2 == 1) pack_fma(e, eo_coef<T, A, I, 0, 2>(eo), xp[A::C / 2]);
 }
 // The R outputs of one or more even-odd contractions: rows(i, e, o) adds the
 // sums of row i, put(i, y) receives output i (i an integral constant).
@@ -753,7 +773,8 @@ __global__ __launch_bounds__(kThreads) void k_cell_sf(Geo g, T hq, const T* __re
                                                       const double* __restrict__ xn,
                                                       const double* __restrict__ xcol,
                                                       T* __restrict__ Yc, int B,
-                                                      double* __restrict__ dotp) {
+                                                      double* __restrict__ dotp,
+                                                      const double* __restrict__ eo) {
   constexpr int N1 = P + 1, Q = P + 3, N2 = N1 * N1, NE = N2 * N1, Q2 = Q * Q;
   constexpr int QP = Q | 1, N1P = N1 | 1, NV = CG / VC;
   // x-line items per thread (lines x column packs over the work-group)
@@ -766,11 +787,39 @@ __global__ __launch_bounds__(kThreads) void k_cell_sf(Geo g, T hq, const T* __re
   // regions, each [entry][NV] packs:
   //   A  [(c,b)][N1P]      ka -> k1 -> yk            (kinetic chain, in place)
   //   Bk [(c,b)][N1P]      kb -> k2                  (stages 1-3)
-  //   D  [c][Q][QP]        t1 -> t2 -> s2 -> s1      (stages 4-8, over Bk)
+  //   D  [c][Q][QS]        t1 -> t2 -> s2 -> s1      (stages 4-8, over Bk)
+  // SWZ: D unpadded (QS = Q = 8), entry (c, y, x) at slot x ^ y of its line
+  constexpr bool SWZ = sizeof(T) == 8 && Q == 8;
+  constexpr int QS = SWZ ? Q : QP;
   __shared__ __align__(16) pk rA[N2 * N1P * NV];
-  __shared__ pk rBD[cmax(N2 * N1P, N1 * Q * QP) * NV];
+  __shared__ __align__(16) pk rBD[cmax(N2 * N1P, N1 * Q * QS) * NV];
   pk* const rB = rBD;
-  pk* const rD = rBD;
+  // Byte offsets into D of entry (c, y, x), column pack jv, from a per-item
+  // base and the one index that is a compile-time constant at the access
+  // (so that the rest folds into the LDS instruction's offset field):
+  //   dy(c, x, jv) then at_y(base, y);  dx(c, y, jv) then at_x(base, x);
+  //   dc(y, x, jv) then at_c(base, c).
+  // With SWZ the slot x ^ y is an XOR of the offset's slot bits (the slot
+  // field, NV sizeof(pk) bytes per slot and 8 slots, holds no other bits).
+  constexpr uint32_t SB = NV * sizeof(pk);
+  const auto dy = [](uint32_t c, uint32_t x, uint32_t jv) -> uint32_t {
+    return (c * Q * QS + x) * SB + jv * (uint32_t)sizeof(pk);
+  };
+  const auto at_y = [](uint32_t base, uint32_t y) -> uint32_t {
+    return SWZ ? (base ^ (y * SB)) + y * QS * SB : base + y * QS * SB;
+  };
+  const auto dx = [](uint32_t c, uint32_t y, uint32_t jv) -> uint32_t {
+    return ((c * Q + y) * QS + (SWZ ? y : 0u)) * SB + jv * (uint32_t)sizeof(pk);
+  };
+  const auto at_x = [](uint32_t base, uint32_t x) -> uint32_t {
+    return SWZ ? base ^ (x * SB) : base + x * SB;
+  };
+  const auto dc = [](uint32_t y, uint32_t x, uint32_t jv) -> uint32_t {
+    return (y * QS + (SWZ ? (x ^ y) : x)) * SB + jv * (uint32_t)sizeof(pk);
+  };
+  const auto at_c = [](uint32_t base, uint32_t c) -> uint32_t { return base + c * Q * QS * SB; };
+  char* const rDb = reinterpret_cast<char*>(rBD);
+  const auto dref = [rDb](uint32_t off) -> pk& { return *reinterpret_cast<pk*>(rDb + off); };
   // (cell, column group) pairs dealt to the 8 XCDs in contiguous runs, the
   // groups of one cell at consecutive slots of one XCD, so that they share
   // that XCD's L2 for the cell's rows of X and of the cell scratch (each group
@@ -862,10 +911,10 @@ __global__ __launch_bounds__(kThreads) void k_cell_sf(Geo g, T hq, const T* __re
     }
     eo_split(v, keep_p[ki], keep_m[ki]);
     eo_out<N1, T, VC>(
-        [&](auto i, pk& s, pk& d) { eo_row<T, TM, IDFT_IC(i)>(keep_p[ki], keep_m[ki], s, d); },
+        [&](auto i, pk& s, pk& d) { eo_row<T, TM, IDFT_IC(i)>(keep_p[ki], keep_m[ki], s, d, eo); },
         [&](auto a2, const pk& r) { rA[(ln * N1P + IDFT_IC(a2)) * NV + jv] = r; });
     eo_out<N1, T, VC>(
-        [&](auto i, pk& s, pk& d) { eo_row<T, TD, IDFT_IC(i)>(keep_p[ki], keep_m[ki], s, d); },
+        [&](auto i, pk& s, pk& d) { eo_row<T, TD, IDFT_IC(i)>(keep_p[ki], keep_m[ki], s, d, eo); },
         [&](auto a2, const pk& r) { rB[(ln * N1P + IDFT_IC(a2)) * NV + jv] = r; });
   }
   __syncthreads();
@@ -883,12 +932,12 @@ __global__ __launch_bounds__(kThreads) void k_cell_sf(Geo g, T hq, const T* __re
     eo_split(va, ap, am);
     eo_split(vb, bp, bm);
     eo_out<N1, T, VC>(
-        [&](auto i, pk& s, pk& d) { eo_row<T, TM, IDFT_IC(i)>(ap, am, s, d); },
+        [&](auto i, pk& s, pk& d) { eo_row<T, TM, IDFT_IC(i)>(ap, am, s, d, eo); },
         [&](auto b2, const pk& r) { rA[((c * N1 + IDFT_IC(b2)) * N1P + a) * NV + jv] = r; });
     eo_out<N1, T, VC>(
         [&](auto i, pk& s, pk& d) {
-          eo_row<T, TD, IDFT_IC(i)>(ap, am, s, d);
-          eo_row<T, TM, IDFT_IC(i)>(bp, bm, s, d);
+          eo_row<T, TD, IDFT_IC(i)>(ap, am, s, d, eo);
+          eo_row<T, TM, IDFT_IC(i)>(bp, bm, s, d, eo);
         },
         [&](auto b2, const pk& r) { rB[((c * N1 + IDFT_IC(b2)) * N1P + a) * NV + jv] = r; });
   }
@@ -907,8 +956,8 @@ __global__ __launch_bounds__(kThreads) void k_cell_sf(Geo g, T hq, const T* __re
     eo_split(v2, p2, m2);
     eo_out<N1, T, VC>(
         [&](auto i, pk& s, pk& d) {
-          eo_row<T, TD, IDFT_IC(i)>(p1, m1, s, d);
-          eo_row<T, TM, IDFT_IC(i)>(p2, m2, s, d);
+          eo_row<T, TD, IDFT_IC(i)>(p1, m1, s, d, eo);
+          eo_row<T, TM, IDFT_IC(i)>(p2, m2, s, d, eo);
         },
         [&](auto c2, const pk& r) { rA[((IDFT_IC(c2) * N1 + b) * N1P + a) * NV + jv] = r; });
   }
@@ -921,22 +970,24 @@ __global__ __launch_bounds__(kThreads) void k_cell_sf(Geo g, T hq, const T* __re
     if (it >= N2 * NV) break;
     const int jv = it % NV, ln = it / NV;
     const int b = ln % N1, c = ln / N1;
+    const uint32_t base = dx(c, b, jv);
     eo_out<Q, T, VC>(
-        [&](auto i, pk& s, pk& d) { eo_row<T, TL, IDFT_IC(i)>(keep_p[ki], keep_m[ki], s, d); },
-        [&](auto q, const pk& r) { rD[((c * Q + b) * QP + IDFT_IC(q)) * NV + jv] = r; });
+        [&](auto i, pk& s, pk& d) { eo_row<T, TL, IDFT_IC(i)>(keep_p[ki], keep_m[ki], s, d, eo); },
+        [&](auto q, const pk& r) { dref(at_x(base, IDFT_IC(q))) = r; });
   }
   __syncthreads();
   // stage 5: y lines (c, qx): t2[c][qy][qx] = Lq t1, in place (over t1)
   for (int it = tid; it < N1 * Q * NV; it += kThreads) {
     const int jv = it % NV, ln = it / NV;
     const int qx = ln % Q, c = ln / Q;
+    const uint32_t base = dy(c, qx, jv);
     pk v[N1], vp[(N1 + 1) / 2], vm[N1 / 2];
 #pragma unroll
-    for (int b = 0; b < N1; ++b) v[b] = rD[((c * Q + b) * QP + qx) * NV + jv];
+    for (int b = 0; b < N1; ++b) v[b] = dref(at_y(base, b));
     eo_split(v, vp, vm);
     eo_out<Q, T, VC>(
-        [&](auto i, pk& s, pk& d) { eo_row<T, TL, IDFT_IC(i)>(vp, vm, s, d); },
-        [&](auto qy, const pk& r) { rD[((c * Q + IDFT_IC(qy)) * QP + qx) * NV + jv] = r; });
+        [&](auto i, pk& s, pk& d) { eo_row<T, TL, IDFT_IC(i)>(vp, vm, s, d, eo); },
+        [&](auto qy, const pk& r) { dref(at_y(base, IDFT_IC(qy))) = r; });
   }
   __syncthreads();
   // stage 6: z lines (qy, qx) through the quadrature weights, in place:
@@ -947,33 +998,35 @@ __global__ __launch_bounds__(kThreads) void k_cell_sf(Geo g, T hq, const T* __re
     if (it >= Q2 * NV) break;
     const int jv = it % NV, ln = it / NV;  // ln = qy * Q + qx
     const int qx = ln % Q, qy = ln / Q;
+    const uint32_t base = dc(qy, qx, jv);
     pk v[N1], vp[(N1 + 1) / 2], vm[N1 / 2];
 #pragma unroll
-    for (int c = 0; c < N1; ++c) v[c] = rD[((c * Q + qy) * QP + qx) * NV + jv];
+    for (int c = 0; c < N1; ++c) v[c] = dref(at_c(base, c));
     eo_split(v, vp, vm);
     pk u[Q], up[(Q + 1) / 2], um[Q / 2];
-    eo_out<Q, T, VC>([&](auto i, pk& s, pk& d) { eo_row<T, TL, IDFT_IC(i)>(vp, vm, s, d); },
+    eo_out<Q, T, VC>([&](auto i, pk& s, pk& d) { eo_row<T, TL, IDFT_IC(i)>(vp, vm, s, d, eo); },
                      [&](auto qz, const pk& r) {
                        const T wq = wpre[k6][IDFT_IC(qz)];
 #pragma unroll
                        for (int uu = 0; uu < VC; ++uu) u[IDFT_IC(qz)].v[uu] = r.v[uu] * wq;
                      });
     eo_split(u, up, um);
-    eo_out<N1, T, VC>([&](auto i, pk& s, pk& d) { eo_row<T, TLt, IDFT_IC(i)>(up, um, s, d); },
-                      [&](auto c, const pk& r) { rD[((IDFT_IC(c) * Q + qy) * QP + qx) * NV + jv] = r; });
+    eo_out<N1, T, VC>([&](auto i, pk& s, pk& d) { eo_row<T, TLt, IDFT_IC(i)>(up, um, s, d, eo); },
+                      [&](auto c, const pk& r) { dref(at_c(base, IDFT_IC(c))) = r; });
   }
   __syncthreads();
   // stage 7: y lines back (c, qx), in place: s1[c][b][qx] = Lq^T s2 (over s2)
   for (int it = tid; it < N1 * Q * NV; it += kThreads) {
     const int jv = it % NV, ln = it / NV;
     const int qx = ln % Q, c = ln / Q;
+    const uint32_t base = dy(c, qx, jv);
     pk v[Q], vp[(Q + 1) / 2], vm[Q / 2];
 #pragma unroll
-    for (int qy = 0; qy < Q; ++qy) v[qy] = rD[((c * Q + qy) * QP + qx) * NV + jv];
+    for (int qy = 0; qy < Q; ++qy) v[qy] = dref(at_y(base, qy));
     eo_split(v, vp, vm);
     eo_out<N1, T, VC>(
-        [&](auto i, pk& s, pk& d) { eo_row<T, TLt, IDFT_IC(i)>(vp, vm, s, d); },
-        [&](auto b, const pk& r) { rD[((c * Q + IDFT_IC(b)) * QP + qx) * NV + jv] = r; });
+        [&](auto i, pk& s, pk& d) { eo_row<T, TLt, IDFT_IC(i)>(vp, vm, s, d, eo); },
+        [&](auto b, const pk& r) { dref(at_y(base, IDFT_IC(b))) = r; });
   }
   __syncthreads();
   // stage 8: x lines back (c, b) plus the kinetic part, into the cell scratch;
@@ -990,12 +1043,13 @@ __global__ __launch_bounds__(kThreads) void k_cell_sf(Geo g, T hq, const T* __re
     const int jv = it % NV, ln = it / NV;
     if (jv * VC >= bw) continue;
     const int b = ln % N1, c = ln / N1;
+    const uint32_t base = dx(c, b, jv);
     pk v[Q], vp[(Q + 1) / 2], vm[Q / 2];
 #pragma unroll
-    for (int qx = 0; qx < Q; ++qx) v[qx] = rD[((c * Q + b) * QP + qx) * NV + jv];
+    for (int qx = 0; qx < Q; ++qx) v[qx] = dref(at_x(base, qx));
     eo_split(v, vp, vm);
     T* yp = Yc + (size_t)((e * NE + ln * N1) * ub + c0 + jv * VC);
-    eo_out<N1, T, VC>([&](auto i, pk& s, pk& d) { eo_row<T, TLt, IDFT_IC(i)>(vp, vm, s, d); },
+    eo_out<N1, T, VC>([&](auto i, pk& s, pk& d) { eo_row<T, TLt, IDFT_IC(i)>(vp, vm, s, d, eo); },
                       [&](auto a, const pk& r) {
                         constexpr int A = IDFT_IC(a);
                         const pk kin = rA[(ln * N1P + A) * NV + jv];
@@ -1789,31 +1843,43 @@ __global__ __launch_bounds__(kThreads) void k_sum_partials(int nblk, int B,
   if (threadIdx.x == 0) out[j] = s;
 }
 
+// Sums of contiguous rows: out[j] = sum of partial[j nblk .. (j + 1) nblk).
+__global__ __launch_bounds__(kRedThreads) void k_sum_rows(int nblk, const double* __restrict__ partial,
+                                                           double* __restrict__ out) {
+  __shared__ double red[kRedThreads / 64];
+  const int j = blockIdx.x;
+  const double s = block_sum_contig<kRedThreads>(partial + (int64_t)j * nblk, nblk, red);
+  if (threadIdx.x == 0) out[j] = s;
+}
+
 // Gram partials: block (bx, by) accumulates the entries ij in
-// [by * 256, by * 256 + 256) of G over a strided set of 32-node chunks whose
-// rows of X and Y are staged in LDS.
-constexpr int kGramChunk = 32;
-__global__ __launch_bounds__(kThreads) void k_gram(int64_t N, int B, const double* __restrict__ X,
+// [by * 256, by * 256 + 256) of G = X^T Y over the chunks of `rows` nodes
+// bx, bx + gridDim.x, ..., whose rows of X and Y are staged in LDS (the loads
+// of a chunk issued together).  Partials column-major, partial[ij nblk + bx],
+// so that each entry's reduction reads contiguous memory.
+__global__ __launch_bounds__(kThreads) void k_gram(int64_t N, int B, int rows,
+                                                   const double* __restrict__ X,
                                                    const double* __restrict__ Y,
                                                    double* __restrict__ partial) {
   extern __shared__ double lds[];
   double* xs = lds;
-  double* ys = lds + kGramChunk * B;
+  double* ys = lds + rows * B;
   const int tid = threadIdx.x;
-  const int BB = B * B;
+  const int BB = B * B, RB = rows * B;
   const int ij = blockIdx.y * kThreads + tid;
   const int i = ij / B, j = ij % B;
   double acc = 0.0;
-  for (int64_t n0 = (int64_t)blockIdx.x * kGramChunk; n0 < N; n0 += (int64_t)gridDim.x * kGramChunk) {
-    const int cnt = (int)((N - n0) < kGramChunk ? (N - n0) : kGramChunk);
+  for (int64_t n0 = (int64_t)blockIdx.x * rows; n0 < N; n0 += (int64_t)gridDim.x * rows) {
+    const int cnt = (int)((N - n0) < rows ? (N - n0) : rows);
+    const int ce = cnt * B;  // entries of this chunk (contiguous in X and Y)
     __syncthreads();
     // four entries per thread per round, their loads issued together
-    for (int t0 = tid; t0 < kGramChunk * B; t0 += 4 * kThreads) {
+    for (int t0 = tid; t0 < RB; t0 += 4 * kThreads) {
       double xv[4], yv[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const int t = t0 + u * kThreads;
-        const bool ok = t < kGramChunk * B && t / B < cnt;
+        const bool ok = t < ce;
         const int64_t g = ok ? n0 * B + t : 0;
         xv[u] = X[g];
         yv[u] = Y[g];
@@ -1822,7 +1888,7 @@ __global__ __launch_bounds__(kThreads) void k_gram(int64_t N, int B, const doubl
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const int t = t0 + u * kThreads;
-        if (t < kGramChunk * B) {
+        if (t < RB) {
           xs[t] = xv[u];
           ys[t] = yv[u];
         }
@@ -1832,25 +1898,50 @@ __global__ __launch_bounds__(kThreads) void k_gram(int64_t N, int B, const doubl
     if (ij < BB)
       for (int r = 0; r < cnt; ++r) acc += xs[r * B + i] * ys[r * B + j];
   }
-  if (ij < BB) partial[(int64_t)blockIdx.x * BB + ij] = acc;
+  if (ij < BB) partial[(int64_t)ij * gridDim.x + blockIdx.x] = acc;
 }
 
-// Y = beta Y + ns[n] (X Q)[n]  (X != Y)
-__global__ void k_right_mult(uint32_t NB, uint32_t B, const double* __restrict__ X,
-                             const double* __restrict__ Q, double* __restrict__ Y,
-                             const double* __restrict__ ns, double beta) {
-  extern __shared__ double qs[];
-  for (uint32_t i = threadIdx.x; i < B * B; i += blockDim.x) qs[i] = Q[i];
-  __syncthreads();
-  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < NB; t += gridDim.x * blockDim.x) {
-    const uint32_t n = t / B;
-    const uint32_t j = t - n * B;
-    const double* xr = X + (size_t)n * B;
-    double s = 0.0;
-#pragma unroll 4
-    for (uint32_t i = 0; i < B; ++i) s += xr[i] * qs[i * B + j];
-    if (ns) s *= ns[n];
-    Y[t] = beta != 0.0 ? beta * Y[t] + s : s;
+// Y = beta Y + ns[n] (X Q)[n]  (X != Y) over tiles of `rows` nodes: the
+// tile's rows of X (contiguous) are staged in LDS with their loads issued
+// together, then each thread forms the outputs t, t + 256, ... of the tile
+// (node t / B, column t % B) from LDS, and the tile is written contiguously.
+__global__ __launch_bounds__(kThreads) void k_right_mult(uint32_t N, uint32_t B, uint32_t rows,
+                                                         const double* __restrict__ X,
+                                                         const double* __restrict__ Q,
+                                                         double* __restrict__ Y,
+                                                         const double* __restrict__ ns,
+                                                         double beta) {
+  extern __shared__ double lds[];
+  double* qs = lds;
+  double* xt = lds + B * B;
+  const uint32_t tid = threadIdx.x, RB = rows * B;
+  for (uint32_t i = tid; i < B * B; i += kThreads) qs[i] = Q[i];
+  for (uint32_t n0 = blockIdx.x * rows; n0 < N; n0 += gridDim.x * rows) {
+    const uint32_t cnt = min(rows, N - n0), ce = cnt * B;
+    const uint32_t base = n0 * B;  // N B < 2^32 (checked by the host)
+    __syncthreads();  // the previous tile's outputs have read xt
+    for (uint32_t t0 = tid; t0 < ce; t0 += 4 * kThreads) {
+      double xv[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t t = t0 + u * kThreads;
+        xv[u] = X[t < ce ? base + t : base];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t t = t0 + u * kThreads;
+        if (t < ce) xt[t] = xv[u];
+      }
+    }
+    __syncthreads();
+    for (uint32_t t = tid; t < ce; t += kThreads) {
+      const uint32_t r = t / B, j = t - r * B;
+      const double* xr = xt + r * B;
+      double s = 0.0;
+      for (uint32_t i = 0; i < B; ++i) s += xr[i] * qs[i * B + j];
+      if (ns) s *= ns[n0 + r];
+      Y[base + t] = beta != 0.0 ? beta * Y[base + t] + s : s;
+    }
   }
 }
 
@@ -1959,12 +2050,12 @@ __global__ __launch_bounds__(kThreads) void k_minres_alpha(int B, int nblk,
 // alpha of a fused step: v.(H~ v) from the per-cell dots x_e.(H_e x_e)
 // (x = S v the cell product's input) plus the node terms of the step's
 // output, v.(cvn D r2 + cr1 r1) = cv (cvn Q2 + cr1 Q3) with v = cv D r2.
-__global__ __launch_bounds__(kThreads) void k_minres_alpha_cell(int B, int ncell,
-                                                                 const double* __restrict__ cd,
-                                                                 double* __restrict__ st) {
-  __shared__ double red[kThreads];
+__global__ __launch_bounds__(kRedThreads) void k_minres_alpha_cell(int B, int ncell,
+                                                                    const double* __restrict__ cd,
+                                                                    double* __restrict__ st) {
+  __shared__ double red[kRedThreads / 64];
   const int j = blockIdx.x;
-  const double s = block_sum_contig(cd + (int64_t)j * ncell, ncell, red);
+  const double s = block_sum_contig<kRedThreads>(cd + (int64_t)j * ncell, ncell, red);
   if (threadIdx.x != 0) return;
   const double cv = st[MS_CV * B + j];
   const double a = s + cv * (st[MS_CVN * B + j] * st[MS_Q2 * B + j] +
@@ -2041,21 +2132,21 @@ __global__ __launch_bounds__(kThreads) void k_minres_lanczos(int64_t N, int B,
   }
 }
 
-__global__ __launch_bounds__(kThreads) void k_minres_givens(int B, int nblk,
-                                                             const double* __restrict__ partial,
-                                                             const double* __restrict__ negEig,
-                                                             double* __restrict__ st, int nq) {
-  __shared__ double red[kThreads];
+__global__ __launch_bounds__(kRedThreads) void k_minres_givens(int B, int nblk,
+                                                                const double* __restrict__ partial,
+                                                                const double* __restrict__ negEig,
+                                                                double* __restrict__ st, int nq) {
+  __shared__ double red[kRedThreads / 64];
   const int j = blockIdx.x;
   double s, q2 = 0.0, q3 = 0.0;
   if (nq >= 3) {  // fused steps: [k][column][block] partials
-    s = block_sum_contig(partial + (int64_t)j * nblk, nblk, red);
+    s = block_sum_contig<kRedThreads>(partial + (int64_t)j * nblk, nblk, red);
     __syncthreads();
-    q2 = block_sum_contig(partial + ((int64_t)B + j) * nblk, nblk, red);
+    q2 = block_sum_contig<kRedThreads>(partial + ((int64_t)B + j) * nblk, nblk, red);
     __syncthreads();
-    q3 = block_sum_contig(partial + ((int64_t)2 * B + j) * nblk, nblk, red);
+    q3 = block_sum_contig<kRedThreads>(partial + ((int64_t)2 * B + j) * nblk, nblk, red);
   } else {
-    s = block_sum_col(partial, nblk, B, j, red);
+    s = block_sum_col<kRedThreads>(partial, nblk, B, j, red);
   }
   if (threadIdx.x != 0) return;
   double* S = st;
@@ -2241,13 +2332,15 @@ void launch_build_hamiltonian(const Geo& g, const DevTables& t, double hh, const
 
 // Columns per work-group (CG) and per item (VC) of the sum-factorised
 // kernel built into the library: CG in {2, 4, 8} (its LDS holds 684 CG
-// values at p = 5, 43.8 KB in FP64 at CG = 8), one column per item.  The kernel is
+// values at p = 5, 636 CG in FP64: 40.7 KB at CG = 8), one column per item.  The kernel is
 // written for VC columns per item; wider items and 16-column groups measured
 // slower on every mesh tried and are not instantiated.
 bool sf_cols_supported(int cols, bool) { return cols == 2 || cols == 4 || cols == 8; }
 bool sf_vc_supported(int cols, int vc, bool fp64) {
   return sf_cols_supported(cols, fp64) && vc == 1;
 }
+
+static const double* sf_eo_table(int p);
 
 template <typename T, typename TV, int P, int CG>
 static void cell_sf_cg(const Geo& g, const CellOp<T>& op, const T* S, const TV* X,
@@ -2259,6 +2352,7 @@ static void cell_sf_cg(const Geo& g, const CellOp<T>& op, const T* S, const TV* 
   check_u32(8 * ((npair + 7) / 8));
   dim3 grid((unsigned)(8 * ((npair + 7) / 8))), block(kThreads);
   const T hq = (T)op.kin_scale;
+  const double* eo = sf_eo_table(P);
   // per-cell dots only with node-scaled inputs (the MINRES steps)
   if (dotp && !xn) throw Error(IDFT_EINVAL, "per-cell dots need a node-scaled input");
   // node-scaled inputs (MINRES) come as FP64 vectors; FP32 vectors never
@@ -2266,14 +2360,14 @@ static void cell_sf_cg(const Geo& g, const CellOp<T>& op, const T* S, const TV* 
   if constexpr (std::is_same<TV, double>::value) {
     if (xn) {
       k_cell_sf<T, TV, P, CG, 1, true><<<grid, block, 0, s>>>(g, hq, op.W, S, X, xn, xcol, Yc, B,
-                                                               dotp);
+                                                               dotp, eo);
       return;
     }
   } else {
     if (xn) throw Error(IDFT_EINVAL, "node-scaled input needs FP64 vectors");
   }
   k_cell_sf<T, TV, P, CG, 1, false><<<grid, block, 0, s>>>(g, hq, op.W, S, X, nullptr, xcol, Yc, B,
-                                                             nullptr);
+                                                             nullptr, eo);
 }
 
 template <typename T, typename TV, int P>
@@ -2327,6 +2421,10 @@ static void cell_gemm_p(const Geo& g, const CellOp<T>& op, const T* S, const TV*
   }
 }
 
+// Device addresses of the even-odd tables kEo2..kEo5, per device, set by
+// upload_sf_constants (once per device, before any cell product there).
+static std::atomic<const double*> g_eo_table[64][4];
+
 void upload_sf_constants() {
   static std::mutex mu;
   static bool done[64] = {};
@@ -2343,7 +2441,23 @@ void upload_sf_constants() {
   IDFT_HIP(hipMemcpyToSymbol(HIP_SYMBOL(kEo3), &t3, sizeof(t3), 0, hipMemcpyHostToDevice));
   IDFT_HIP(hipMemcpyToSymbol(HIP_SYMBOL(kEo4), &t4, sizeof(t4), 0, hipMemcpyHostToDevice));
   IDFT_HIP(hipMemcpyToSymbol(HIP_SYMBOL(kEo5), &t5, sizeof(t5), 0, hipMemcpyHostToDevice));
+  void* a[4] = {};
+  IDFT_HIP(hipGetSymbolAddress(&a[0], HIP_SYMBOL(kEo2)));
+  IDFT_HIP(hipGetSymbolAddress(&a[1], HIP_SYMBOL(kEo3)));
+  IDFT_HIP(hipGetSymbolAddress(&a[2], HIP_SYMBOL(kEo4)));
+  IDFT_HIP(hipGetSymbolAddress(&a[3], HIP_SYMBOL(kEo5)));
+  for (int k = 0; k < 4; ++k) g_eo_table[dev][k].store(static_cast<const double*>(a[k]));
   done[dev] = true;
+}
+
+// The even-odd table of degree p on the current device.
+static const double* sf_eo_table(int p) {
+  int dev = 0;
+  IDFT_HIP(hipGetDevice(&dev));
+  const double* t = (dev >= 0 && dev < 64 && p >= 2 && p <= 5) ? g_eo_table[dev][p - 2].load()
+                                                                : nullptr;
+  if (!t) throw Error(IDFT_ESTATE, "sum-factorised tables not uploaded on this device");
+  return t;
 }
 
 int64_t cell_scratch_elems(const Geo& g, int B) { return g.ncell * (int64_t)ne_of(g.p) * B; }
@@ -2586,28 +2700,40 @@ void launch_sum_partials(int nblk, int B, const double* partial, double* out, hi
   IDFT_HIP(hipGetLastError());
 }
 
+// Nodes per LDS tile of the Gram kernel: 64 for narrow blocks, about 2048
+// entries per tile for wide ones.
+static int tall_tile_rows(int B) { return std::max(8, std::min(64, 2048 / std::max(B, 1))); }
+
+// Gram partial rows: enough work-groups to fill the CUs (up to 2048, 8 per
+// CU), at most 2^21 partial values.
 int gram_partial_blocks(int64_t N, int B) {
-  (void)B;
-  int64_t g = (N + kGramChunk - 1) / kGramChunk;
-  if (g > 512) g = 512;
+  const int rows = tall_tile_rows(B);
+  const int64_t cap = std::max<int64_t>(64, std::min<int64_t>(2048, (int64_t(1) << 21) / ((int64_t)B * B)));
+  const int64_t g = std::min<int64_t>((N + rows - 1) / rows, cap);
   return (int)(g < 1 ? 1 : g);
 }
 
 void launch_gram(int64_t N, int B, const double* X, const double* Y, double* partial, double* G,
                  hipStream_t s) {
-  if (B > kMaxCols) throw Error(IDFT_EINVAL, "block too wide for Gram kernel");
-  const int nb = gram_partial_blocks(N, B);
+  if (B < 1 || B > kMaxCols) throw Error(IDFT_EINVAL, "block width out of range for the Gram kernel");
+  const int nb = gram_partial_blocks(N, B), rows = tall_tile_rows(B);
   dim3 grid(nb, (B * B + kThreads - 1) / kThreads);
-  k_gram<<<grid, kThreads, 2 * kGramChunk * B * sizeof(double), s>>>(N, B, X, Y, partial);
+  k_gram<<<grid, kThreads, 2 * rows * B * sizeof(double), s>>>(N, B, rows, X, Y, partial);
   IDFT_HIP(hipGetLastError());
-  launch_sum_partials(nb, B * B, partial, G, s);
+  k_sum_rows<<<B * B, kRedThreads, 0, s>>>(nb, partial, G);
+  IDFT_HIP(hipGetLastError());
 }
 
 void launch_right_mult(int64_t N, int B, const double* X, const double* Q, double* Y,
                        const double* ns, double beta, hipStream_t s) {
+  if (B < 1 || B > kMaxCols) throw Error(IDFT_EINVAL, "block width out of range");
   check_u32(N * B);
-  k_right_mult<<<grid_for(N * B), kThreads, B * B * sizeof(double), s>>>((uint32_t)(N * B),
-                                                                         (uint32_t)B, X, Q, Y, ns, beta);
+  // Q and the tile within 64 KB of LDS (8 nodes per tile at the widest block)
+  const int rows = std::max(8, std::min(64, (8192 - B * B) / B));
+  const int64_t tiles = (N + rows - 1) / rows;
+  const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(tiles, 4096));
+  k_right_mult<<<grid, kThreads, ((size_t)B * B + (size_t)rows * B) * sizeof(double), s>>>(
+      (uint32_t)N, (uint32_t)B, (uint32_t)rows, X, Q, Y, ns, beta);
   IDFT_HIP(hipGetLastError());
 }
 
@@ -2731,7 +2857,7 @@ int launch_minres_start_dots(int64_t N, int B, const double* r, const double* D,
   return nb;
 }
 void launch_minres_alpha_cell(int B, int ncell, const double* cell_dots, double* st, hipStream_t s) {
-  k_minres_alpha_cell<<<B, kThreads, 0, s>>>(B, ncell, cell_dots, st);
+  k_minres_alpha_cell<<<B, kRedThreads, 0, s>>>(B, ncell, cell_dots, st);
   IDFT_HIP(hipGetLastError());
 }
 void launch_minres_alpha(int B, int nblk, const double* partial, double* st, hipStream_t s) {
@@ -2752,7 +2878,7 @@ template int launch_minres_lanczos<double>(int64_t, int, const double*, const do
                                            const double*, const double*, double*, hipStream_t);
 void launch_minres_givens(int B, int nblk, const double* partial, const double* negEig, double* st,
                           hipStream_t s, int nq) {
-  k_minres_givens<<<B, kThreads, 0, s>>>(B, nblk, partial, negEig, st, nq);
+  k_minres_givens<<<B, kRedThreads, 0, s>>>(B, nblk, partial, negEig, st, nq);
   IDFT_HIP(hipGetLastError());
 }
 template <typename T>
