@@ -74,7 +74,7 @@ def sf_flops_per_cell_column(p):
 # profiles/cell_gemm_inverse_24c_p5_pmc.md (2 16 4 0 and 1 16 0 1).
 MEASURED_TRAFFIC = {
     ("inverse_24c_p5", "k_cell_sf<float,float,5,4,1,false>"): 276.3e6,
-    ("inverse_24c_p5", "k_cell_sf<double,double,5,4,1,false>"): 552.4e6,
+    ("inverse_24c_p5", "k_cell_sf<double,double,5,4,1,false>"): 552.5e6,
     ("batched_40c_p4", "k_cell_sf<float,float,4,8,1,false>"): 4245.0e6,
     ("batched_40c_p4", "k_cell_sf<double,double,4,8,1,false>"): 9262.7e6,
     ("inverse_24c_p5", "k_cell_gemm<float,float,5,1>"): 2613e6,
