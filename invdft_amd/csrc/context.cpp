@@ -46,6 +46,10 @@ struct AdjointState {
   DevBuf<int> dpairs;
   DevBuf<double> docc4, dnegEig, uJxW;
   std::vector<double> Mu;
+  // sum_q u JxW psi_i psi_j of the pairs, for the current psi and u (set by
+  // computeRhs, reused by distributeX)
+  std::vector<double> pairQ;
+  bool havePairQ = false;
 };
 
 // Work buffers of one MINRES precision.
@@ -665,7 +669,11 @@ struct idft_ctx {
 
   void compute_mu(const std::vector<double>& occv) {
     const int Bb = adj.B;
-    std::vector<double> v = pair_quad_sum(adj.psi->p(), adj.psi->p(), Bb, adj.uJxW.p, t.nq);
+    if (!adj.havePairQ) {
+      adj.pairQ = pair_quad_sum(adj.psi->p(), adj.psi->p(), Bb, adj.uJxW.p, t.nq);
+      adj.havePairQ = true;
+    }
+    const std::vector<double>& v = adj.pairQ;
     adj.Mu.assign((size_t)Bb * Bb, 0.0);
     for (size_t k = 0; k < v.size(); ++k) {
       const int i = adj.pairs[2 * k], j = adj.pairs[2 * k + 1];
@@ -679,6 +687,7 @@ struct idft_ctx {
     if (!psi || psi->ncols != b || b <= 0) throw Error(IDFT_EINVAL, "psi block size mismatch");
     AdjointState& A = adj;
     A.B = b; A.psi = psi; A.fermi = fermiE; A.x = nullptr;
+    A.havePairQ = false;
     zero_boundary(psi->p(), b);
     A.occ.assign(occv, occv + b);
     std::vector<double> occ4(b);
@@ -710,17 +719,37 @@ struct idft_ctx {
     if (x && x->ncols != b) throw Error(IDFT_EINVAL, "block size mismatch");
     A.x = x;
     const double* psi = A.psi->p();
-    compute_mu(A.occ);
-    // rhs = -2 sqrt(M) psi Mu   (column i: -2 sum_k Mu[k][i] M^{1/2} psi_k)
-    std::vector<double> C((size_t)b * b);
-    for (size_t k = 0; k < C.size(); ++k) C[k] = -2.0 * A.Mu[k];
-    right_mult(psi, C, rhs, b, sqrtM.p, 0.0);
-    // rhs += 4 f M^{-1/2} R psi, R applied on the fly as N^T diag(u JxW) N
+    // T2 = R psi, R = N^T diag(u JxW) N assembled over the cells
     T2buf.resize(NB(b));
     double* T2 = T2buf.p;
     launch_lincomb<double>(t.N, b, T2, 0.0, nullptr, 0.0, nullptr, nullptr, 0.0, nullptr, nullptr, s);
     for (int colour = 0; colour < 8; ++colour)
       launch_quad_operator(g, dt, psi, b, A.uJxW.p, T2, colour, s);
+    // the pairs' quadrature sums sum_q u JxW psi_i psi_j = psi_i . (R psi)_j
+    // (R assembled from the same quadrature), before T2's boundary rows are
+    // cleared; more pairs than the node-dot kernel takes (256) are summed at
+    // the quadrature points by compute_mu instead
+    const int np = (int)A.pairs.size() / 2;
+    A.havePairQ = false;
+    if (np > 0 && np <= 256) {
+      partial.resize((size_t)1024 * np);
+      small.resize(std::max(np, 16));
+      const int nb = launch_pairdot_partial(t.N, b, psi, T2, nullptr, A.dpairs.p, np, partial.p, s);
+      launch_sum_partials(nb, np, partial.p, small.p, s);
+      A.pairQ.resize(np);
+      small.download(A.pairQ.data(), np, s);
+      sync();
+      A.havePairQ = true;
+    } else if (np == 0) {
+      A.pairQ.clear();
+      A.havePairQ = true;
+    }
+    compute_mu(A.occ);
+    // rhs = -2 sqrt(M) psi Mu   (column i: -2 sum_k Mu[k][i] M^{1/2} psi_k)
+    std::vector<double> C((size_t)b * b);
+    for (size_t k = 0; k < C.size(); ++k) C[k] = -2.0 * A.Mu[k];
+    right_mult(psi, C, rhs, b, sqrtM.p, 0.0);
+    // rhs += 4 f M^{-1/2} R psi
     zero_boundary(T2, b);
     launch_lincomb<double>(t.N, b, rhs, 1.0, T2, 1.0, A.docc4.p, nullptr, 0.0, nullptr, invSqrtM.p, s);
     zero_boundary(rhs, b);
