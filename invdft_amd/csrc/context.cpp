@@ -920,8 +920,10 @@ struct idft_ctx {
     // Lanczos vector v = cv D r2 is formed on the fly (not stored).
     // Fused step: 4 launches and N*bc streams 1 (cell product) + 3 (gather:
     // r2, r1 in, r1 out) + 5 (the previous step's update inside the gather:
-    // w, w1 and x in, w and x out; r1 is read anyway).  A run of steps ends
-    // with finish_run(), the last step's update on its own (6 streams).
+    // w, w1 and x in, w and x out; r1 is read anyway).  The last step's
+    // update of a run carries over into the next run's first gather; only
+    // before a compaction and at the end does finish_run() apply it on its own
+    // (6 streams), since those read x.
     bool pending = false;  // a step whose update has not run yet
     auto fused_step = [&] {
       const double* cv = mst.p + (size_t)MS_CV * bc;
@@ -987,11 +989,12 @@ struct idft_ctx {
     const bool useGraph = d.minres_graph != 0 && check % 2 == 0;
     Yc64.resize(cell_scratch_elems(g, b));
     if (canRelax) Yc32.resize(cell_scratch_elems(g, b));
-    hipGraphExec_t execs[2] = {nullptr, nullptr};
+    // graphs per (operator precision, update pending at the run's start)
+    hipGraphExec_t execs[4] = {nullptr, nullptr, nullptr, nullptr};
     struct ExecGuard {
       hipGraphExec_t* e;
       void drop() {
-        for (int i = 0; i < 2; ++i)
+        for (int i = 0; i < 4; ++i)
           if (e[i]) { (void)hipGraphExecDestroy(e[i]); e[i] = nullptr; }
       }
       ~ExecGuard() { drop(); }
@@ -1044,7 +1047,10 @@ struct idft_ctx {
       std::vector<int> open;
       for (int j = 0; j < bc; ++j)
         if (!flags[j]) open.push_back(j);
-      if (open.empty() || k >= maxit) break;
+      if (open.empty() || k >= maxit) {
+        finish_run();
+        break;
+      }
       // repack once enough columns have finished, keeping the width even
       // (one finished column rides along, inert: its coefficients are zero)
       // so that the block kernels move two columns per 16-byte access
@@ -1061,6 +1067,7 @@ struct idft_ctx {
           kflags[i] = flags[keep[i]];
           pb[i] = phibar[keep[i]];
         }
+        finish_run();  // x complete before it is copied out
         compact(keep);
         st.minres_compactions++;
         if (verbose)
@@ -1081,14 +1088,16 @@ struct idft_ctx {
       }
       const int n = std::min(check, maxit - k);
       st.minres_column_steps += (int64_t)n * bc;
-      if (useGraph && n == check) {
-        hipGraphExec_t& exec = execs[f32op ? 1 : 0];
+      // a run of fused steps that starts without a pending update (the
+      // first after the start or a compaction) is launched directly; the
+      // runs after it, which all start with one, replay a graph
+      if (useGraph && n == check && (pending || !fused)) {
+        hipGraphExec_t& exec = execs[(f32op ? 1 : 0) + (pending ? 2 : 0)];
         if (!exec) {
           hipGraph_t graph = nullptr;
           IDFT_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
           try {
             for (int i = 0; i < n; ++i) step();
-            finish_run();
           } catch (...) {
             (void)hipStreamEndCapture(s, &graph);
             if (graph) (void)hipGraphDestroy(graph);
@@ -1098,18 +1107,21 @@ struct idft_ctx {
           const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
           (void)hipGraphDestroy(graph);
           IDFT_HIP(ie);
-        } else if (f32op) {
-          // the capture counted its n steps; replays count theirs here
-          st.applies_fp32 += n;
-          st.apply_cols_fp32 += (int64_t)n * bc;
         } else {
-          st.applies_fp64 += n;
-          st.apply_cols_fp64 += (int64_t)n * bc;
+          // the capture counted its n steps and left an update pending;
+          // replays count theirs here and leave one too
+          if (f32op) {
+            st.applies_fp32 += n;
+            st.apply_cols_fp32 += (int64_t)n * bc;
+          } else {
+            st.applies_fp64 += n;
+            st.apply_cols_fp64 += (int64_t)n * bc;
+          }
+          if (fused) pending = true;
         }
         IDFT_HIP(hipGraphLaunch(exec, s));
       } else {
         for (int i = 0; i < n; ++i) step();
-        finish_run();
       }
       k += n;
     }
