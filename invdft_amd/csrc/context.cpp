@@ -14,6 +14,8 @@
 #include <type_traits>
 
 #include "../../include/idft_setup.h"
+#include <rocblas/rocblas.h>
+
 #include "common.h"
 #include "setup.h"
 #include "sf_tables.h"
@@ -114,6 +116,25 @@ struct idft_ctx {
   DevBuf<int> mflags, msel;
   idft_mv mvPsi, mvX;
 
+  // rocBLAS handle on s (created on first use), for the Gram and X Q products
+  // of blocks of kBlasCols columns or more: there they are GEMMs bound by
+  // arithmetic (N B^2 multiply-adds), which the library's DGEMM runs on the
+  // matrix cores.  Atomics off: the split-K reductions sum in a fixed order.
+  static constexpr int kBlasCols = 24;
+  rocblas_handle blas = nullptr;
+  rocblas_handle blas_handle() {
+    if (!blas) {
+      if (rocblas_create_handle(&blas) != rocblas_status_success) {
+        blas = nullptr;
+        throw Error(IDFT_EHIP, "rocblas_create_handle failed");
+      }
+      if (rocblas_set_stream(blas, s) != rocblas_status_success ||
+          rocblas_set_atomics_mode(blas, rocblas_atomics_not_allowed) != rocblas_status_success)
+        throw Error(IDFT_EHIP, "rocblas handle set-up failed");
+    }
+    return blas;
+  }
+
   void sync() { IDFT_HIP(hipStreamSynchronize(s)); }
   size_t NB(int b) const { return (size_t)t.N * b; }
 
@@ -192,19 +213,61 @@ struct idft_ctx {
     sync();
     return out;
   }
+  // G[i][j] = sum_n a[n][i] b2[n][j] (row-major B x B)
   std::vector<double> gram(const double* a, const double* b2, int b) {
-    partial.resize((size_t)gram_partial_blocks(t.N, b) * b * b);
     small.resize((size_t)b * b);
-    launch_gram(t.N, b, a, b2, partial.p, small.p, s);
+    if (b >= kBlasCols) {
+      // column-major: a and b2 are B x N (leading dimension B), and
+      // G^T = b2 a^T is G read row-major.  The node sum is split into nk
+      // chunks of kc nodes (one batched DGEMM, one B x B partial each, plus
+      // one for the remainder), then the partials are summed in order: a
+      // single product would leave the long reduction to few work-groups.
+      const int64_t klen = std::min<int64_t>(8192, t.N);
+      const int64_t nk = t.N / klen, rem = t.N - nk * klen;
+      const int64_t bb = (int64_t)b * b;
+      partial.resize((size_t)(nk + 1) * bb);
+      const double one = 1.0, zero = 0.0;
+      if (rocblas_dgemm_strided_batched(blas_handle(), rocblas_operation_none,
+                                        rocblas_operation_transpose, b, b, (rocblas_int)klen, &one,
+                                        b2, b, (rocblas_stride)(klen * b), a, b,
+                                        (rocblas_stride)(klen * b), &zero, partial.p, b,
+                                        (rocblas_stride)bb, (rocblas_int)nk) !=
+          rocblas_status_success)
+        throw Error(IDFT_EHIP, "rocblas_dgemm_strided_batched failed (Gram)");
+      int np = (int)nk;
+      if (rem > 0) {
+        const size_t off = (size_t)(nk * klen) * b;
+        if (rocblas_dgemm(blas_handle(), rocblas_operation_none, rocblas_operation_transpose, b, b,
+                          (rocblas_int)rem, &one, b2 + off, b, a + off, b, &zero,
+                          partial.p + (size_t)nk * bb, b) != rocblas_status_success)
+          throw Error(IDFT_EHIP, "rocblas_dgemm failed (Gram remainder)");
+        ++np;
+      }
+      launch_sum_partials(np, (int)bb, partial.p, small.p, s);
+    } else {
+      partial.resize((size_t)gram_partial_blocks(t.N, b) * b * b);
+      launch_gram(t.N, b, a, b2, partial.p, small.p, s);
+    }
     std::vector<double> G((size_t)b * b);
     small.download(G.data(), G.size(), s);
     sync();
     return G;
   }
+  // out = beta out + diag(ns) a Q  (Q row-major B x B)
   void right_mult(const double* a, const std::vector<double>& Q, double* out, int b,
                   const double* ns = nullptr, double beta = 0.0) {
     small2.resize(Q.size());
     small2.upload(Q.data(), Q.size(), s);
+    if (b >= kBlasCols && !ns) {
+      // column-major: out (B x N) = Q^T a, and Q row-major is Q^T
+      // column-major
+      const double one = 1.0;
+      if (rocblas_dgemm(blas_handle(), rocblas_operation_none, rocblas_operation_none, b,
+                        (rocblas_int)t.N, b, &one, small2.p, b, a, b, &beta, out, b) !=
+          rocblas_status_success)
+        throw Error(IDFT_EHIP, "rocblas_dgemm failed (X Q)");
+      return;
+    }
     launch_right_mult(t.N, b, a, small2.p, out, ns, beta, s);
   }
   void zero_boundary(double* v, int b) { launch_zero_rows(t.N, b, v, bnd.p, s); }
@@ -1730,6 +1793,7 @@ int idft_create(const idft_desc* d, const double* pot_base_q, const double* rho_
 void idft_destroy(idft_ctx* c) {
   if (!c) return;
   if (c->s) (void)hipStreamSynchronize(c->s);
+  if (c->blas) (void)rocblas_destroy_handle(c->blas);
   hipStream_t s = c->s;
   delete c;
   if (s) (void)hipStreamDestroy(s);

@@ -95,6 +95,32 @@ def test_eigensolve_density(glib, olib):
     assert _subspace_gap(g.eigenvectors()[:, :nocc], o.eigenvectors()[:, :nocc]) < 1e-8
 
 
+@pytest.mark.parametrize("prec", [abi.FP64, abi.MIXED])
+@pytest.mark.parametrize("n_cells", [3, 6])
+def test_eigensolve_wide_block(glib, olib, prec, n_cells):
+    """32 states: the Gram and X Q products of orthonormalisation and
+    Rayleigh-Ritz run as library DGEMMs from 24 columns on (the Gram's node
+    sum in batched chunks of 8192 nodes plus a remainder: 2197 nodes at 3^3
+    cells are one short chunk, 15625 at 6^3 one chunk and a remainder); the
+    lowest 8 eigenvalues (the solver's convergence test covers the occupied
+    states and a few above them, not the top of the block), occupancies and
+    density against the oracle's."""
+    _say("eigensolve_wide", prec, n_cells)
+    prob = small_problem(olib, n_eig=32, n_cells=n_cells)
+    g = prob.context(glib, precision=prec)
+    o = prob.context(olib)
+    for c in (g, o):
+        c.set_potential(prob.vxc0)
+    eg, rg, _ = g.eigensolve(1e-10, 80)
+    eo, ro, _ = o.eigensolve(1e-10, 80)
+    assert rg < 1e-9 and ro < 1e-9
+    assert np.max(np.abs(eg[:8] - eo[:8])) < 1e-9
+    occg, _ = g.occupancies()
+    occo, _ = o.occupancies()
+    assert np.max(np.abs(occg - occo)) < 1e-8
+    assert rel(g.density(), o.density()) < 1e-8
+
+
 def _adjoint_setup(ctx, prob, psi, occ, eig, u, pairs, fermi):
     B = psi.shape[1]
     mpsi = ctx.mv(B, psi)
