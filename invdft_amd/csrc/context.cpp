@@ -219,9 +219,10 @@ struct idft_ctx {
     if (b >= kBlasCols) {
       // column-major: a and b2 are B x N (leading dimension B), and
       // G^T = b2 a^T is G read row-major.  The node sum is split into nk
-      // chunks of kc nodes (one batched DGEMM, one B x B partial each, plus
-      // one for the remainder), then the partials are summed in order: a
-      // single product would leave the long reduction to few work-groups.
+      // chunks of klen (8192) nodes (one batched DGEMM, one B x B partial
+      // each, plus one for the remainder), then the partials are summed in
+      // order: a single product would leave the long reduction to few
+      // work-groups.
       const int64_t klen = std::min<int64_t>(8192, t.N);
       const int64_t nk = t.N / klen, rem = t.N - nk * klen;
       const int64_t bb = (int64_t)b * b;
@@ -1171,8 +1172,9 @@ struct idft_ctx {
           (void)hipGraphDestroy(graph);
           IDFT_HIP(ie);
         } else {
-          // the capture counted its n steps and left an update pending;
-          // replays count theirs here and leave one too
+          // the capture counted its n steps; replays count theirs here (and,
+          // with fused steps, leave the last update pending as the capture
+          // did)
           if (f32op) {
             st.applies_fp32 += n;
             st.apply_cols_fp32 += (int64_t)n * bc;

@@ -557,7 +557,7 @@ __global__ __launch_bounds__(kSymmThreads) void k_cell_symm(Geo g, const T* __re
 // and puts the lines that one stage-6 item reads 512 B apart, so that the
 // compiler pairs those accesses (ds_read2st64 / ds_write2st64).  It also
 // takes 20.4 KB instead of 21.9 KB per work-group, which does not raise the
-// occupancy: the kernel's 70 registers allow 7 waves per SIMD either way (with
+// occupancy: the kernel's 71 registers allow 7 waves per SIMD either way (with
 // the stage-6 weights loaded after stage 3 it fits 64 registers and 8
 // work-groups per CU, but measured slower: the weights' latency shows).  The kinetic
 // chain runs first and the x-line input stays in registers for the potential
