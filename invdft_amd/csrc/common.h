@@ -228,6 +228,13 @@ template <typename T>
 void launch_lincomb3(int64_t N, int B, T* Y, double beta, const T* X, double a, const double* ca,
                      const T* Z, double b, const double* cb, const T* W, double c,
                      const double* cc, const double* ns, hipStream_t s);
+// Y = a ca[j] X + b cb[j] Z in FP64, rounded once to Y's type (FP64 X and Z
+// into FP32 Y; FP64 X and FP32 Z into FP64 Y).
+void launch_lincomb_d2f(int64_t N, int B, float* Y, const double* X, double a, const double* ca,
+                        const double* Z, double b, const double* cb, hipStream_t s);
+void launch_lincomb_df2d(int64_t N, int B, double* Y, const double* X, double a,
+                         const double* ca, const float* Z, double b, const double* cb,
+                         hipStream_t s);
 void launch_convert_d2f(int64_t n, float* y, const double* x, hipStream_t s);
 void launch_convert_f2d(int64_t n, double* y, const float* x, hipStream_t s);
 int coldot_blocks(int64_t N, int B);

@@ -555,8 +555,7 @@ struct idft_ctx {
     // R = HX - X diag(eig)
     small2.resize(b);
     small2.upload(eig.data(), b, s);
-    launch_lincomb<double>(t.N, b, X2.p, 0.0, HX.p, 1.0, nullptr, X.p, -1.0, small2.p, nullptr, s);
-    launch_convert_d2f((int64_t)n, Rf.p, X2.p, s);
+    launch_lincomb_d2f(t.N, b, Rf.p, HX.p, 1.0, nullptr, X.p, -1.0, small2.p, s);
     const double e = 0.5 * (bb - a), c = 0.5 * (bb + a);
     double sigma = e / (a0 - c);
     const double sigma1 = sigma, tau = 2.0 / sigma1;
@@ -588,9 +587,7 @@ struct idft_ctx {
       float* tmp = dm; dm = dk; dk = dn; dn = tmp;
       sigma = s2;
     }
-    launch_convert_f2d((int64_t)n, X2.p, dk, s);
-    launch_lincomb<double>(t.N, b, X.p, 0.0, X.p, 1.0, ydev.p + (size_t)deg * b, X2.p, 1.0, nullptr,
-                           nullptr, s);
+    launch_lincomb_df2d(t.N, b, X.p, X.p, 1.0, ydev.p + (size_t)deg * b, dk, 1.0, nullptr, s);
   }
 
   // Two sweeps of Cholesky-QR (SVQB when the Gram matrix is not positive definite).

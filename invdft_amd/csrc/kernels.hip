@@ -1796,6 +1796,23 @@ __global__ void k_lincomb(uint32_t NB, uint32_t B, T* __restrict__ Y, double bet
   }
 }
 
+// Y = a ca[j] X + b cb[j] Z with operands and result of different types
+// (null coefficient pointers read as 1), in FP64 arithmetic and rounded once:
+// the same expression as k_lincomb, so that a conversion pass before or after
+// it is not needed.  Y may be X (each entry is read, then written, by one
+// thread).  N * B < 2^32 (checked by the host).
+template <typename TY, typename TX, typename TZ>
+__global__ void k_lincomb_mixed(uint32_t NB, uint32_t B, TY* Y, const TX* X, double a,
+                                const double* __restrict__ ca,
+                                const TZ* __restrict__ Z, double b, const double* __restrict__ cb) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < NB; i += gridDim.x * blockDim.x) {
+    const uint32_t n = i / B, j = i - n * B;
+    double v = a * (ca ? ca[j] : 1.0) * (double)X[i];
+    v += b * (cb ? cb[j] : 1.0) * (double)Z[i];
+    Y[i] = (TY)v;
+  }
+}
+
 __global__ void k_d2f(int64_t n, float* __restrict__ y, const double* __restrict__ x) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
@@ -2665,6 +2682,21 @@ template void launch_lincomb3<double>(int64_t, int, double*, double, const doubl
                                       const double*, double, const double*, const double*,
                                       hipStream_t);
 
+void launch_lincomb_d2f(int64_t N, int B, float* Y, const double* X, double a, const double* ca,
+                        const double* Z, double b, const double* cb, hipStream_t s) {
+  check_u32(N * B);
+  k_lincomb_mixed<float, double, double><<<grid_for(N * B), kThreads, 0, s>>>(
+      (uint32_t)(N * B), (uint32_t)B, Y, X, a, ca, Z, b, cb);
+  IDFT_HIP(hipGetLastError());
+}
+void launch_lincomb_df2d(int64_t N, int B, double* Y, const double* X, double a,
+                         const double* ca, const float* Z, double b, const double* cb,
+                         hipStream_t s) {
+  check_u32(N * B);
+  k_lincomb_mixed<double, double, float><<<grid_for(N * B), kThreads, 0, s>>>(
+      (uint32_t)(N * B), (uint32_t)B, Y, X, a, ca, Z, b, cb);
+  IDFT_HIP(hipGetLastError());
+}
 void launch_convert_d2f(int64_t n, float* y, const double* x, hipStream_t s) {
   k_d2f<<<grid_for(n), kThreads, 0, s>>>(n, y, x);
   IDFT_HIP(hipGetLastError());
