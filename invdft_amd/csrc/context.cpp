@@ -574,14 +574,14 @@ struct idft_ctx {
     ydev.resize(ys.size());
     ydev.upload(ys.data(), ys.size(), s);
     float *dm = F0.p, *dk = F1.p, *dn = F2.p;
-    launch_lincomb<float>(t.N, b, dm, 0.0, nullptr, 0.0, nullptr, nullptr, 0.0, nullptr, nullptr, s);
     launch_lincomb<float>(t.N, b, dk, 0.0, Rf.p, sigma1 / e, nullptr, nullptr, 0.0, nullptr, nullptr, s);
     for (int k = 1; k < deg; ++k) {
       const double s2 = 1.0 / (tau - sigma), f = 2.0 * s2 / e, gk = sigma * s2;
       ApplyEpilogue<float> ep;
       ep.Y = dn; ep.alpha = f;
       ep.X = dk; ep.a = -f * c;
-      ep.Z = dm; ep.b = -gk;
+      // D_0 = 0: the first step has no -g D_{k-1} term (dm holds nothing yet)
+      ep.Z = k > 1 ? dm : nullptr; ep.b = -gk;
       ep.W = Rf.p; ep.c = f; ep.cc = ydev.p + (size_t)k * b;
       apply<float>(dk, b, ep);
       float* tmp = dm; dm = dk; dk = dn; dn = tmp;
